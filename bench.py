@@ -1,0 +1,234 @@
+"""CP-ALS benchmark on one MI355X: 512^3 float32 tensor, rank 32.
+
+A step is one CP-ALS sweep (every mode's fused MTTKRP, FP64 Gram-Hadamard
+Cholesky solve and the FP64 fit) over the resident 512^3 tensor.  The timed
+configuration is the fast path (bf16x3 MTTKRP on MFMA with the dimension-tree
+sweep).  ``value`` is MTTKRP-equivalent throughput of the timed sweeps:
+3 MTTKRPs x 537,067,520 algorithmic bytes (tensor read once, factors and
+output) per sweep / wall time.
+
+Also reported on the same JSON line:
+  * time_to_fit: wall time until the relative error is within 1e-4 of the FP64
+    oracle's error after 100 sweeps (tests/golden/cp512_oracle.npz), for plain
+    FP32 sweeps and for the fast path, and their ratio;
+  * roofline: the dominant kernel's algorithmic bytes / average duration
+    (HIP events on the launch stream) against 8 TB/s;
+  * cpu_baseline: the numpy restatement of the reference path (unfold copy,
+    materialised Khatri-Rao, np.dot, Hadamard of Grams, solve) timed on the
+    host for one full sweep in float32.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CP-ALS time-to-fit 512^3 R=32: bf16x3+learned vs plain FP32; MTTKRP GB/s"
+SHAPE = (512, 512, 512)
+RANK = 32
+MTTKRP_BYTES = 512 ** 3 * 4 + (512 + 512) * RANK * 4 + 512 * RANK * 4  # 537,067,520
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--quick", action="store_true", help="skip time-to-fit and the CPU baseline")
+    p.add_argument("--fit-sweeps", type=int, default=100)
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_time_ms(fn, iters, torch):
+    """Average duration of fn() on the current stream with HIP events."""
+    fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(iters):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / iters
+
+
+def time_to_fit(engine, n_sweeps, target_err, torch):
+    """Run n sweeps, one event per sweep; return (seconds to reach target,
+    sweeps needed, errors)."""
+    dev = engine.dev
+    errs = torch.zeros(n_sweeps, dtype=torch.float64, device=dev)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(n_sweeps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record()
+    for s in range(n_sweeps):
+        engine.sweeps(1, errs, s, False)
+        evs[s + 1].record()
+    torch.cuda.synchronize()
+    e = errs.cpu().numpy()
+    hit = np.nonzero(e <= target_err)[0]
+    if len(hit) == 0:
+        return None, None, e
+    s = int(hit[0])
+    return evs[0].elapsed_time(evs[s + 1]) / 1e3, s + 1, e
+
+
+def cpu_baseline(x32):
+    """One float32 ALS sweep of the numpy restatement, median of 3 after a
+    warm-up."""
+    from threadpoolctl import threadpool_limits
+    from oracle import tensorly_np as T
+    cores = os.cpu_count() or 1
+    rng = np.random.RandomState(1)
+    factors = [rng.random_sample((s, RANK)).astype(np.float32) for s in SHAPE]
+    w = np.ones(RANK, dtype=np.float32)
+
+    def sweep():
+        for mode in range(3):
+            V = np.ones((RANK, RANK), dtype=np.float32)
+            for i, f in enumerate(factors):
+                if i != mode:
+                    V = V * (f.T @ f)
+            m = T.unfolding_dot_khatri_rao(x32, (w, factors), mode)
+            factors[mode] = np.linalg.solve(V.T, m.T).T.astype(np.float32)
+        T.error_calc(x32, 1.0, w, factors, mttkrp=m)
+
+    with threadpool_limits(cores):
+        sweep()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sweep()
+            ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return {"value": round(3 * MTTKRP_BYTES / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"one float32 CP-ALS sweep (3 MTTKRPs + solves + fit) on the 512^3 tensor, median of 3 "
+                      f"after 1 warm-up: {t:.3f} s/sweep, numpy {np.__version__}, {platform.processor() or 'cpu'}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only "
+              "(see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import tensorly_amd as tla
+    from tensorly_amd import synthetic, _lib
+    from tensorly_amd.decomposition._cp import CpAlsEngine
+    from tensorly_amd.tenalg import mttkrp_partial
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    x_host, _ = synthetic.cp_512_rank32()
+    log(f"[bench] data generated in {time.time() - t0:.1f}s")
+    X = torch.from_numpy(x_host).to(dev)
+    rng = np.random.RandomState(1)
+    init = [torch.tensor(rng.random_sample((s, RANK)), dtype=torch.float32, device=dev) for s in SHAPE]
+    ones = torch.ones(RANK, dtype=torch.float32, device=dev)
+
+    # ---- timed sweeps: fast path --------------------------------------------------------
+    fast = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    errs = torch.zeros(max(args.steps, 1), dtype=torch.float64, device=dev)
+    fast.sweeps(args.warmup, None, 0, False)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    fast.sweeps(args.steps, errs, 0, False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = 3 * MTTKRP_BYTES * args.steps / elapsed / 1e9
+    log(f"[bench] {args.steps} sweeps: {ms_per_step:.3f} ms/sweep, {value:.1f} GB/s MTTKRP-equivalent "
+        f"(dimtree={fast.dimtree})")
+
+    # ---- per-kernel roofline (HIP events on the launch stream) -------------------------------
+    facs = fast.factors
+    kern = {}
+    with tla.precision("bf16x3"):
+        kern["mttkrp_partial_modes01_bf16x3"] = kernel_time_ms(
+            lambda: mttkrp_partial(X, (None, facs), (0, 2)), 20, torch)
+        kern["mttkrp_mode2_bf16x3"] = kernel_time_ms(lambda: tla.unfolding_dot_khatri_rao(X, (None, facs), 2), 20,
+                                                     torch)
+        kern["mttkrp_mode0_bf16x3"] = kernel_time_ms(lambda: tla.unfolding_dot_khatri_rao(X, (None, facs), 0), 20,
+                                                     torch)
+    with tla.precision("fp32"):
+        for m in range(3):
+            kern[f"mttkrp_mode{m}_fp32"] = kernel_time_ms(
+                lambda m=m: tla.unfolding_dot_khatri_rao(X, (None, facs), m), 20, torch)
+    partial_bytes = 512 ** 3 * 4 + 512 * RANK * 4 + 512 * 512 * RANK * 4
+    kernels = {k: {"ms": round(v, 4),
+                   "GB/s": round((partial_bytes if "partial" in k else MTTKRP_BYTES) / (v * 1e-3) / 1e9, 1)}
+               for k, v in kern.items()}
+    dom = max(("mttkrp_partial_modes01_bf16x3", "mttkrp_mode2_bf16x3"), key=lambda k: kern[k])
+    dom_bytes = partial_bytes if "partial" in dom else MTTKRP_BYTES
+    achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
+    log(f"[bench] kernels: {json.dumps(kernels)}")
+
+    # ---- time to fit ----------------------------------------------------------------------------------
+    ttf = None
+    if not args.quick:
+        oracle_path = os.path.join(ROOT, "tests", "golden", "cp512_oracle.npz")
+        if os.path.exists(oracle_path):
+            ref = np.load(oracle_path)["errors"]
+            target = float(ref[-1]) + 1e-4
+            res = {}
+            for name, prec, dimtree in (("fp32_plain", "fp32", False), ("bf16x3_dimtree", "bf16x3", True)):
+                eng = CpAlsEngine(X, ones, init, precision=prec, dimtree=dimtree)
+                t, n, e = time_to_fit(eng, args.fit_sweeps, target, torch)
+                res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
+                eng.close()
+                log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
+            speed = None
+            if res["fp32_plain"]["seconds"] and res["bf16x3_dimtree"]["seconds"]:
+                speed = res["fp32_plain"]["seconds"] / res["bf16x3_dimtree"]["seconds"]
+            ttf = {"target_error": target, "oracle_error_100_sweeps": float(ref[-1]), **res,
+                   "speedup_vs_fp32": speed}
+    fast.close()
+
+    cpu = None
+    if not (args.no_cpu or args.quick):
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(x_host)
+        log(f"[bench] CPU baseline {cpu['value']} GB/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3",
+        "data": "synthetic",
+        "config": {"workload": "cp_512_rank32: 512^3 float32 CP model, rank 32, U[0,1) factors, 1% noise; "
+                               "one CP-ALS sweep per step (bf16x3 MFMA MTTKRP, dimension tree, FP64 solve+fit)",
+                   "rank": RANK, "shape": list(SHAPE), "sweep": "bf16x3+dimtree", "dimtree_active": fast.dimtree},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernels": kernels,
+        "time_to_fit": ttf,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

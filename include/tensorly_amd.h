@@ -1,0 +1,213 @@
+/*
+ * tensorly_amd -- C ABI of the MI355X (gfx950) CP-ALS / Tucker-HOOI kernels.
+ *
+ * Plain pointers and sizes only.  Every pointer argument named like a tensor
+ * is a DEVICE pointer; shape / pointer-list arguments are HOST arrays.
+ * `stream` is a hipStream_t (NULL = default stream).  Entry points never
+ * allocate device memory: scratch space ("workspace") is provided by the
+ * caller, sized by the matching *_workspace_size() query.  Every entry point
+ * returns TLA_OK (0) or an error code; tla_last_error() (thread-local) gives
+ * the message.
+ *
+ * Layout contract (the numpy / pytorch backends' own): dense tensors are
+ * C-order with the last mode contiguous; factor matrices are row-major
+ * (I_m x R); `unfold(X, n)` columns and Khatri-Rao rows run row-major over the
+ * remaining modes in increasing order (tensorly/base.py:39-53,
+ * tensorly/tenalg/core_tenalg/_khatri_rao.py:94-105).
+ *
+ * Interfaces replaced are cited as reference file:line (tensorly 0.9.0).
+ */
+#ifndef TENSORLY_AMD_H
+#define TENSORLY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TLA_OK 0
+#define TLA_ERR_ARG 1         /* bad shape / argument: tensorly raises ValueError */
+#define TLA_ERR_HIP 2         /* HIP runtime error */
+#define TLA_ERR_UNSUPPORTED 3 /* combination not implemented */
+#define TLA_ERR_NUMERIC 4     /* singular system / non-finite result */
+
+/* element types */
+#define TLA_F32 0
+#define TLA_F64 1
+#define TLA_BF16 2 /* bf16 storage copy of a float32 tensor (MTTKRP only) */
+
+/* arithmetic of the tensor contraction */
+#define TLA_PREC_FP32 0   /* FP32 MFMA, FP32 accumulation */
+#define TLA_PREC_BF16 1   /* BF16 MFMA, FP32 accumulation */
+#define TLA_PREC_BF16X3 2 /* split hi+lo BF16, 3 MFMAs per product, FP32 accum. */
+#define TLA_PREC_VALU 3   /* plain vector ALU in the tensor's own type */
+
+const char* tla_last_error(void);
+int tla_abi_version(void);
+
+/* ---------------------------------------------------------------------
+ * MTTKRP:  out(I_mode x R) = unfold(X, mode) @ khatri_rao(factors, weights,
+ * skip_matrix=mode), with the Khatri-Rao product never materialised.
+ * Replaces tensorly/tenalg/core_tenalg/mttkrp.py:9-50
+ * (unfolding_dot_khatri_rao) and tensorly/tenalg/einsum_tenalg/mttkrp.py:6-39.
+ *   x_dtype  TLA_F32 / TLA_F64 / TLA_BF16; factors, weights and out are F64
+ *            when x_dtype is F64, else F32.
+ *   factors  host array of ndim device pointers (entry `mode` is ignored).
+ *   weights  device pointer of R values, or NULL for ones.
+ * ------------------------------------------------------------------- */
+size_t tla_mttkrp_workspace_size(int x_dtype, int ndim, const int64_t* shape, int rank,
+                                 int mode, int precision);
+int tla_mttkrp(int x_dtype, const void* X, int ndim, const int64_t* shape,
+               const void* const* factors, const void* weights, int rank, int mode,
+               int precision, void* out, void* workspace, size_t workspace_bytes,
+               void* stream);
+
+/* ---------------------------------------------------------------------
+ * Partial MTTKRP (dimension tree): contract every mode except the
+ * contiguous block [keep_begin, keep_end) -- which must be a prefix
+ * (keep_begin == 0) or a suffix (keep_end == ndim) of the modes -- with the
+ * corresponding factor columns.  Output is the (prod kept sizes) x R matrix
+ * Y[q, r] = sum_p X[..] * w_r * prod_{m not kept} A_m[i_m, r].
+ * Used by the CP-ALS dimension-tree sweep; no tensorly counterpart.
+ * ------------------------------------------------------------------- */
+size_t tla_mttkrp_partial_workspace_size(int x_dtype, int ndim, const int64_t* shape,
+                                         int rank, int keep_begin, int keep_end,
+                                         int precision);
+int tla_mttkrp_partial(int x_dtype, const void* X, int ndim, const int64_t* shape,
+                       const void* const* factors, const void* weights, int rank,
+                       int keep_begin, int keep_end, int precision, void* out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Contract a rank-diagonal 3-way array Y (I x J x R) with one factor:
+ * contract_axis 1: out[i,r] = sum_j Y[i,j,r] F[j,r]
+ * contract_axis 0: out[j,r] = sum_i Y[i,j,r] F[i,r]
+ * (second level of the dimension tree). F32 only. */
+size_t tla_rankdiag_contract_workspace_size(int64_t I, int64_t J, int rank, int contract_axis);
+int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const float* F,
+                          int contract_axis, float* out, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Khatri-Rao product (materialised, for the khatri_rao API only).
+ * Replaces tensorly/tenalg/core_tenalg/_khatri_rao.py:9-109.
+ *   matrices host array of n device pointers (rows[i] x R, row-major);
+ *   weights  R values or NULL; mask prod(rows) values or NULL.
+ * ------------------------------------------------------------------- */
+int tla_khatri_rao(int dtype, const void* const* matrices, int n_matrices, const int64_t* rows,
+                   int rank, const void* weights, const void* mask, void* out, void* stream);
+
+/* ---------------------------------------------------------------------
+ * n-mode product (TTM / TTV).  Replaces
+ * tensorly/tenalg/core_tenalg/n_mode_product.py:5-76 (mode_dot).
+ *   Matrix case: U is (J x I_mode) (transpose=0) or (I_mode x J)
+ *   (transpose=1); out has shape[mode] replaced by J.
+ *   Vector case (is_vector=1): U holds I_mode values; out drops the mode.
+ * ------------------------------------------------------------------- */
+int tla_mode_dot(int dtype, const void* X, int ndim, const int64_t* shape, const void* U,
+                 int64_t J, int is_vector, int mode, int transpose, int precision, void* out,
+                 void* stream);
+
+/* ---------------------------------------------------------------------
+ * FP64 Gram G = A^T A of a factor (rows x R), the A_i^T A_i products of
+ * tensorly/decomposition/_cp.py:411-416 and tensorly/cp_tensor.py:635-637.
+ * ------------------------------------------------------------------- */
+size_t tla_gram64_workspace_size(int64_t rows, int rank);
+int tla_gram64(int dtype, const void* A, int64_t rows, int rank, double* G, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * ALS factor update (tensorly/decomposition/_cp.py:411-427):
+ *   V = (prod_{i != mode} G_i + l2_reg I) * (w w^T)          (FP64)
+ *   factor = mttkrp @ inv(V)          (FP64 Cholesky; LU with partial
+ *                                      pivoting if V is not positive definite)
+ * and, fused, the FP64 Gram of the new factor (gram_out) and, if
+ * iprod_out != NULL, <mttkrp, factor> (the error_calc inner product,
+ * _cp.py:222).  grams: host array of ndim device pointers to R x R FP64.
+ * Returns TLA_ERR_NUMERIC only through `status_out` (device int, nullable):
+ * 0 ok, 1 LU fallback used, 2 singular.
+ * ------------------------------------------------------------------- */
+size_t tla_cp_solve_workspace_size(int64_t rows, int rank);
+int tla_cp_solve(int dtype, const double* const* grams, int ndim, int mode,
+                 const void* weights, double l2_reg, const void* mttkrp, int64_t rows, int rank,
+                 void* factor_out, double* gram_out, double* iprod_out, int* status_out,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Squared Frobenius norm with FP64 accumulation (the norm of
+ * tensorly/backend/core.py:714-739, order 2, before the sqrt).
+ * ------------------------------------------------------------------- */
+size_t tla_sqnorm_workspace_size(int64_t n);
+int tla_sqnorm(int dtype, const void* X, int64_t n, double* out, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * CP reconstruction error from Grams (tensorly/decomposition/_cp.py:217-225
+ * with cp_norm, tensorly/cp_tensor.py:614-644):
+ *   err = sqrt(|normx2 + sum((prod_m G_m) * w w^T) - 2 * iprod|)
+ *   errors[slot] = err / sqrt(normx2)
+ * All operands are device FP64 scalars / matrices; weights in `dtype`.
+ * ------------------------------------------------------------------- */
+int tla_cp_error(const double* const* grams, int ndim, const void* weights, int dtype, int rank,
+                 const double* normx2, const double* iprod, double* errors, int slot,
+                 void* stream);
+
+
+/* ---------------------------------------------------------------------
+ * Leading left singular vectors through the FP64 Gram (replaces the LAPACK
+ * SVD of tensorly/tenalg/svd.py:211-235 on the Tucker and init='svd' paths,
+ * called from tensorly/decomposition/_tucker.py:63-79,194-202 and
+ * tensorly/decomposition/_cp.py:72-100).
+ *   tla_mode_gram64: G = unfold(X, mode) unfold(X, mode)^T (I_mode^2 FP64),
+ *                    without forming the unfolding.
+ *   tla_sym_eig:     k leading eigenpairs (descending) of symmetric FP64 G
+ *                    (n x n).  n <= 64: Jacobi; n > 64: subspace iteration
+ *                    with Rayleigh-Ritz, block 64, k <= 56, optional warm
+ *                    start Q0 (n x k0), converged when every leading residual
+ *                    is <= tol * lambda_1 (or after max_iter steps; the step
+ *                    count goes to *iters_out if non-NULL, device int).
+ * ------------------------------------------------------------------- */
+size_t tla_mode_gram64_workspace_size(int ndim, const int64_t* shape, int mode);
+int tla_mode_gram64(int dtype, const void* X, int ndim, const int64_t* shape, int mode, double* G,
+                    void* workspace, size_t workspace_bytes, void* stream);
+size_t tla_sym_eig_workspace_size(int n);
+int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals,
+                double* evecs, double tol, int max_iter, int* iters_out, void* workspace,
+                size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Native CP-ALS sweep driver: the mode loop of
+ * tensorly/decomposition/_cp.py:394-531 (parafac) run from C++.
+ * The handle keeps device pointers to X, the factors (updated in place) and
+ * the weights, plus a carve-up of the caller's workspace (FP64 Grams, ‖X‖²,
+ * the MTTKRP buffer, dimension-tree intermediates and kernel scratch).
+ *   fixed_modes: ndim flags (nullable); the last mode is never fixed.
+ *   dimtree: 1 = share partial contractions between modes (orders 3, 4).
+ * tla_cp_als_refresh computes ‖X‖² (with_norm) and the factor Grams; call it
+ * before the first sweep and after changing factors outside the driver.
+ * tla_cp_als_sweeps runs n sweeps; errors[first_slot + s] (device FP64,
+ * nullable) receives the relative reconstruction error of sweep s.
+ * ------------------------------------------------------------------- */
+size_t tla_cp_als_workspace_size(int x_dtype, int ndim, const int64_t* shape, int rank, int precision,
+                                 int dimtree);
+int tla_cp_als_dimtree_supported(int x_dtype, int ndim, const int64_t* shape, int rank, int precision);
+int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int64_t* shape, int rank,
+                      void* const* factors, void* weights, double l2_reg, int precision, int dimtree,
+                      const int* fixed_modes, void* workspace, size_t workspace_bytes,
+                      void** handle_out);
+int tla_cp_als_dimtree_active(void* handle);
+int tla_cp_als_refresh(void* handle, int with_norm, void* stream);
+int tla_cp_als_set_normx2(void* handle, const double* normx2, void* stream);
+int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int first_slot, int normalize,
+                      void* stream);
+/* copy state to a device buffer: 0 Grams (ndim x R x R FP64), 1 ‖X‖²,
+ * 2 last <M, A> (FP64), 3 last solve status (int) */
+int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t bytes, void* stream);
+void tla_cp_als_destroy(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TENSORLY_AMD_H */

@@ -1,0 +1,691 @@
+"""Plain-numpy FP64 restatement of the reference TensorLy 0.9.0 CP-ALS /
+Tucker-HOOI path.
+
+TEST INFRASTRUCTURE ONLY -- the checker for the HIP path, never the thing
+measured or shipped.  Product code must not import it.
+
+Every function cites the reference file:line whose numerics it follows
+(paths relative to the reference repository root, ``tensorly/...``).  The op
+sequence mirrors the reference so that the CPU baseline in ``bench.py`` times
+the same work (``unfold`` copy, Khatri-Rao materialisation, ``np.dot``,
+Hadamard of Grams, ``np.linalg.solve``, ``error_calc``).
+
+Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
+
+* ``unfold``/``fold``/``khatri_rao``/``mode_dot``/``multi_mode_dot``/
+  ``cp_to_tensor``: the reference's own known-answer tables
+  (``tests/golden/known_answers.json``) plus identities.
+* ``unfolding_dot_khatri_rao`` (+ memory and einsum forms): brute-force
+  quadruple loop on small inputs, and the three forms against each other.
+* ``cp_norm``: brute-force ``||cp_to_tensor||``.  ``cp_normalize``: unit
+  columns and unchanged reconstruction.
+* ``error_calc``: brute-force ``||X - cp_to_tensor||``.
+* ``random_cp`` / ``initialize_cp(init='random')``: the RandomState draw order,
+  checked against the reference run recorded in the trajectory fixtures.
+* ``parafac``, ``initialize_cp(init='svd')``, ``partial_tucker``/``tucker``:
+  per-iteration trajectories recorded from the reference implementation
+  (``tests/golden/trajectories.npz``, written by
+  ``tests/golden/make_trajectories.py``), plus monotone-decrease and
+  least-squares optimality (gradient = 0 after each mode update).
+* ``svd_flip``/``truncated_svd``/``symeig_svd``: LAPACK identities
+  (``U S V = A``, orthonormality) and the sign rule checked on hand-built cases.
+"""
+import math
+import warnings
+
+import numpy as np
+
+__all__ = [
+    "check_random_state", "norm", "unfold", "fold", "khatri_rao",
+    "unfolding_dot_khatri_rao", "unfolding_dot_khatri_rao_memory",
+    "unfolding_dot_khatri_rao_einsum", "mode_dot", "multi_mode_dot",
+    "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
+    "cp_norm", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
+    "svd_interface", "initialize_cp", "error_calc", "parafac",
+    "validate_tucker_rank", "tucker_to_tensor", "initialize_tucker",
+    "partial_tucker", "tucker",
+]
+
+
+# --------------------------------------------------------------------------
+# L1 primitives
+# --------------------------------------------------------------------------
+def check_random_state(seed):
+    """``backend/core.py:162-188``: None -> numpy's global RandomState,
+    int -> fresh ``RandomState(seed)``, RandomState -> itself."""
+    if seed is None:
+        return np.random.mtrand._rand
+    if isinstance(seed, int):
+        return np.random.RandomState(seed)
+    if isinstance(seed, np.random.RandomState):
+        return seed
+    raise ValueError("Seed should be None, int or np.random.RandomState")
+
+
+def norm(tensor, order=2, axis=None):
+    """``backend/core.py:714-739``: l-``order`` norm, ``sqrt(sum(|x|^2))``
+    for order 2."""
+    if axis == ():
+        axis = None
+    a = np.abs(tensor)
+    if order == "inf":
+        return np.max(a, axis=axis)
+    if order == 1:
+        return np.sum(a, axis=axis)
+    if order == 2:
+        return np.sqrt(np.sum(a ** 2, axis=axis))
+    return np.sum(a ** order, axis=axis) ** (1 / order)
+
+
+# --------------------------------------------------------------------------
+# L2 layout: unfold / fold  (base.py:39-79)
+# --------------------------------------------------------------------------
+def unfold(tensor, mode):
+    """``base.py:39-53``: move ``mode`` to the front, then C-order reshape
+    to ``(I_mode, -1)``.  Columns run row-major over the remaining modes in
+    increasing order."""
+    t = np.moveaxis(tensor, mode, 0)
+    return np.reshape(t, (tensor.shape[mode], -1))
+
+
+def fold(unfolded, mode, shape):
+    """``base.py:56-79``: inverse of :func:`unfold`."""
+    full = list(shape)
+    lead = full.pop(mode)
+    full.insert(0, lead)
+    return np.moveaxis(np.reshape(unfolded, full), 0, mode)
+
+
+# --------------------------------------------------------------------------
+# L3 tensor algebra
+# --------------------------------------------------------------------------
+def khatri_rao(matrices, weights=None, skip_matrix=None, mask=None):
+    """``tenalg/core_tenalg/_khatri_rao.py:9-109``.
+
+    Column-wise Kronecker product, first matrix varying slowest.  ``weights``
+    scale the first (non-skipped) matrix (``:95-99``); one matrix is
+    returned unchanged (``:68-69``, weights ignored there, as in the
+    reference); 1-D inputs are treated as single columns (``:71-79``)."""
+    if skip_matrix is not None:
+        matrices = [m for i, m in enumerate(matrices) if i != skip_matrix]
+    if len(matrices) == 1:
+        return matrices[0]
+    if np.ndim(matrices[0]) == 2:
+        ncol = matrices[0].shape[1]
+    else:
+        ncol = 1
+        matrices = [np.reshape(m, (-1, 1)) for m in matrices]
+        warnings.warn("Khatri-rao of a series of vectors instead of matrices. "
+                      "Considering each as a matrix with 1 column.")
+    for i, m in enumerate(matrices):
+        if np.ndim(m) != 2:
+            raise ValueError("All the matrices must have exactly 2 dimensions!"
+                             f"Matrix {i} has dimension {np.ndim(m)} != 2.")
+        if m.shape[1] != ncol:
+            raise ValueError("All matrices must have same number of columns!"
+                             f"Matrix {i} has {m.shape[1]} columns != {ncol}.")
+    acc = matrices[0] if weights is None else matrices[0] * np.reshape(weights, (1, -1))
+    for m in matrices[1:]:
+        acc = np.reshape(acc[:, None, :] * m[None, :, :], (-1, ncol))
+    if mask is not None:
+        acc = acc * np.reshape(mask, (-1, 1))
+    return acc
+
+
+def unfolding_dot_khatri_rao(tensor, cp_tensor, mode):
+    """``tenalg/core_tenalg/mttkrp.py:47-50``: MTTKRP as
+    ``unfold(X, mode) @ conj(khatri_rao(factors, weights, skip=mode))``
+    (KR materialised, one BLAS GEMM)."""
+    weights, factors = cp_tensor
+    kr = khatri_rao(factors, weights=weights, skip_matrix=mode)
+    return np.dot(unfold(tensor, mode), np.conj(kr))
+
+
+def unfolding_dot_khatri_rao_memory(tensor, cp_tensor, mode):
+    """``tenalg/core_tenalg/mttkrp.py:89-104``: one tensor-times-vector
+    chain per rank-one component, weights applied at the end."""
+    weights, factors = cp_tensor
+    rank = factors[0].shape[1]
+    cols = []
+    for r in range(rank):
+        cols.append(multi_mode_dot(tensor, [np.conj(f[:, r]) for f in factors], skip=mode))
+    out = np.stack(cols, axis=1)
+    if weights is not None:
+        out = out * np.reshape(weights, (1, -1))
+    return out
+
+
+def unfolding_dot_khatri_rao_einsum(tensor, cp_tensor, mode):
+    """``tenalg/einsum_tenalg/mttkrp.py:6-39``: the same contraction as a
+    single einsum, ``weights=None`` treated as ones (``:26-27``)."""
+    weights, factors = cp_tensor
+    rank = factors[0].shape[1]
+    if weights is None:
+        weights = np.ones(rank, dtype=factors[0].dtype)
+    letters = "abcdefghijklmnopqstuvwxyz"
+    n = tensor.ndim
+    t_sub = letters[:n]
+    ops = [tensor, weights]
+    subs = [t_sub, "r"]
+    for i, f in enumerate(factors):
+        if i != mode:
+            ops.append(np.conj(f))
+            subs.append(letters[i] + "r")
+    spec = ",".join(subs) + "->" + letters[mode] + "r"
+    return np.einsum(spec, *ops)
+
+
+def mode_dot(tensor, matrix_or_vector, mode, transpose=False):
+    """``tenalg/core_tenalg/n_mode_product.py:5-76``: ``fold(U @ unfold(X,
+    mode))``; a vector removes the mode; ``transpose`` uses the conjugate
+    transpose; ValueError on any shape mismatch."""
+    new_shape = list(tensor.shape)
+    mv = matrix_or_vector
+    if np.ndim(mv) == 2:
+        dim = 0 if transpose else 1
+        if mv.shape[dim] != tensor.shape[mode]:
+            raise ValueError(
+                f"shapes {tensor.shape} and {mv.shape} not aligned in mode-{mode} multiplication: "
+                f"{tensor.shape[mode]} (mode {mode}) != {mv.shape[dim]} (dim 1 of matrix)")
+        if transpose:
+            mv = np.conj(np.transpose(mv))
+        new_shape[mode] = mv.shape[0]
+        is_vec = False
+    elif np.ndim(mv) == 1:
+        if mv.shape[0] != tensor.shape[mode]:
+            raise ValueError(
+                f"shapes {tensor.shape} and {mv.shape} not aligned for mode-{mode} multiplication: "
+                f"{tensor.shape[mode]} (mode {mode}) != {mv.shape[0]} (vector size)")
+        if len(new_shape) > 1:
+            new_shape.pop(mode)
+        else:
+            new_shape = ()
+        is_vec = True
+    else:
+        raise ValueError("Can only take n_mode_product with a vector or a matrix."
+                         f"Provided array of dimension {np.ndim(mv)} not in [1, 2].")
+    res = np.dot(mv, unfold(tensor, mode))
+    if is_vec:
+        return np.reshape(res, new_shape)
+    return fold(res, mode, new_shape)
+
+
+def multi_mode_dot(tensor, matrix_or_vec_list, modes=None, skip=None, transpose=False):
+    """``tenalg/core_tenalg/n_mode_product.py:79-135``: chain of
+    :func:`mode_dot` sorted by mode (stable); ``skip`` is a position in the
+    sorted list; each vector contraction shifts later modes down by one."""
+    if modes is None:
+        modes = range(len(matrix_or_vec_list))
+    pairs = sorted(zip(matrix_or_vec_list, modes), key=lambda p: p[1])
+    res = tensor
+    removed = 0
+    for pos, (mv, mode) in enumerate(pairs):
+        if skip is not None and pos == skip:
+            continue
+        if transpose:
+            res = mode_dot(res, np.conj(np.transpose(mv)), mode - removed)
+        else:
+            res = mode_dot(res, mv, mode - removed)
+        if np.ndim(mv) == 1:
+            removed += 1
+    return res
+
+
+# --------------------------------------------------------------------------
+# CP tensors  (cp_tensor.py)
+# --------------------------------------------------------------------------
+def validate_cp_tensor(cp_tensor):
+    """``cp_tensor.py:163-214``: returns (shape, rank); ValueError on
+    inconsistent factor widths or weights length."""
+    weights, factors = cp_tensor
+    if np.ndim(factors[0]) == 2:
+        rank = int(factors[0].shape[1])
+    elif np.ndim(factors[0]) == 1:
+        rank = 1
+    else:
+        raise ValueError("Got a factor with 3 dimensions but CP factors should be at most 2D, "
+                         "of shape (size, rank).")
+    shape = []
+    for i, f in enumerate(factors):
+        s = f.shape
+        size, r = (s if len(s) == 2 else (s[0], 1))
+        if r != rank:
+            raise ValueError("All the factors of a CP tensor should have the same number of column."
+                             f"However, factors[0].shape[1]={rank} but factors[{i}].shape[1]={r}.")
+        shape.append(size)
+    if weights is not None and np.shape(weights) != (rank,):
+        raise ValueError(f"Given factors for a rank-{rank} CP tensor but len(weights)={np.shape(weights)}.")
+    return tuple(shape), rank
+
+
+def validate_cp_rank(tensor_shape, rank="same", rounding="round"):
+    """``cp_tensor.py:240-275``."""
+    fun = {"ceil": np.ceil, "floor": np.floor, "round": np.round}.get(rounding)
+    if fun is None:
+        raise ValueError(f"Rounding should be of round, floor or ceil, but got {rounding}")
+    if rank == "same":
+        rank = 1.0
+    if isinstance(rank, float):
+        rank = int(fun(np.prod(tensor_shape) * rank / np.sum(tensor_shape)))
+    return rank
+
+
+def cp_normalize(cp_tensor):
+    """``cp_tensor.py:278-321``: weights folded into the first factor, then
+    every factor's columns scaled to unit l2 norm (zero columns kept)."""
+    _, rank = validate_cp_tensor(cp_tensor)
+    weights, factors = cp_tensor
+    if weights is None:
+        weights = np.ones(rank, dtype=factors[0].dtype)
+    out = []
+    for i, f in enumerate(factors):
+        if i == 0:
+            f = f * weights
+            weights = np.ones(rank, dtype=f.dtype)
+        scales = norm(f, axis=0)
+        safe = np.where(scales == 0, np.ones_like(scales), scales)
+        weights = weights * scales
+        out.append(f / np.reshape(safe, (1, -1)))
+    return weights, out
+
+
+def cp_to_tensor(cp_tensor, mask=None):
+    """``cp_tensor.py:433-485``: ``fold((A_0 * w) @ khatri_rao(A_1..)^T, 0)``,
+    or the masked row-sum form."""
+    shape, _ = validate_cp_tensor(cp_tensor)
+    weights, factors = cp_tensor
+    if len(shape) == 1:
+        return np.sum(weights * factors[0], axis=1)
+    if weights is None:
+        weights = 1
+    if mask is None:
+        full = np.dot(factors[0] * weights, np.transpose(khatri_rao(factors, skip_matrix=0)))
+    else:
+        full = np.sum(khatri_rao([factors[0] * weights] + list(factors[1:]), mask=mask), axis=1)
+    return fold(full, 0, shape)
+
+
+def cp_norm(cp_tensor):
+    """``cp_tensor.py:614-644``: ``sqrt(sum((*_m A_m^T A_m) * w w^T))``."""
+    validate_cp_tensor(cp_tensor)
+    weights, factors = cp_tensor
+    r = factors[0].shape[1]
+    acc = np.ones((r, r), dtype=factors[0].dtype)
+    for f in factors:
+        acc = acc * np.dot(np.transpose(f), np.conj(f))
+    if weights is not None:
+        acc = acc * (np.reshape(weights, (-1, 1)) * np.reshape(weights, (1, -1)))
+    return np.sqrt(np.sum(acc))
+
+
+def random_cp(shape, rank, full=False, orthogonal=False, random_state=None,
+              normalise_factors=True, dtype=np.float64):
+    """``random/base.py:64-114``: one ``random_sample((I_m, R))`` draw per mode
+    in mode order, then ones weights; optional QR, full or normalised."""
+    rank = validate_cp_rank(shape, rank)
+    rng = check_random_state(random_state)
+    factors = [np.asarray(rng.random_sample((s, rank)), dtype=dtype) for s in shape]
+    weights = np.ones(rank, dtype=dtype)
+    if orthogonal:
+        factors = [np.linalg.qr(f)[0] for f in factors]
+    if full:
+        return cp_to_tensor((weights, factors))
+    if normalise_factors:
+        return cp_normalize((weights, factors))
+    return weights, factors
+
+
+# --------------------------------------------------------------------------
+# SVD interface  (tenalg/svd.py)
+# --------------------------------------------------------------------------
+def svd_flip(U, V, u_based_decision=True):
+    """``tenalg/svd.py:13-65``: make the largest-|.| entry of every U column
+    (first one on ties, np.argmax) positive; V rows follow."""
+    if u_based_decision:
+        idx = np.argmax(np.abs(U), axis=0)
+        signs = np.sign(U[idx, np.arange(U.shape[1])])
+        U = U * np.conj(signs)
+        if V.shape[0] > U.shape[1]:
+            signs = np.concatenate((signs, np.ones(V.shape[0] - U.shape[1], dtype=V.dtype)))
+        V = V * signs[: V.shape[0]][:, None]
+    else:
+        idx = np.argmax(np.abs(V), axis=1)
+        signs = np.sign(V[np.arange(V.shape[0]), idx])
+        V = V * np.conj(signs[:, None])
+        if U.shape[1] > V.shape[0]:
+            signs = np.concatenate((signs, np.ones(U.shape[1] - V.shape[0], dtype=V.dtype)))
+        U = U * signs[: U.shape[1]]
+    return U, V
+
+
+def _svd_checks(matrix, n_eigenvecs=None):
+    """``tenalg/svd.py:173-208``."""
+    if np.ndim(matrix) != 2:
+        raise ValueError(f"matrix be a matrix. matrix.ndim is {np.ndim(matrix)} != 2")
+    d1, d2 = matrix.shape
+    lo, hi = min(d1, d2), max(d1, d2)
+    if n_eigenvecs is None:
+        n_eigenvecs = hi
+    if n_eigenvecs > hi:
+        warnings.warn(f"Trying to compute SVD with n_eigenvecs={n_eigenvecs}, which is larger "
+                      f"than max(matrix.shape)={hi}. Setting n_eigenvecs to {hi}.")
+        n_eigenvecs = hi
+    return n_eigenvecs, lo, hi
+
+
+def truncated_svd(matrix, n_eigenvecs=None, **kwargs):
+    """``tenalg/svd.py:211-235``: LAPACK SVD (full matrices only when more
+    vectors than min(shape) are asked for), truncated."""
+    n, lo, _ = _svd_checks(matrix, n_eigenvecs)
+    U, S, V = np.linalg.svd(matrix, full_matrices=n > lo)
+    return U[:, :n], S[:n], V[:n, :]
+
+
+def symeig_svd(matrix, n_eigenvecs=None, **kwargs):
+    """``tenalg/svd.py:238-285``: SVD through ``eigh`` of the smaller Gram,
+    eigenvalues clipped at eps, then flipped to descending order."""
+    n, _, _ = _svd_checks(matrix, n_eigenvecs)
+    d1, d2 = matrix.shape
+    if d1 > d2:
+        S, U = np.linalg.eigh(np.dot(matrix, np.transpose(matrix)))
+        S = np.sqrt(np.clip(S, np.finfo(S.dtype).eps, None))
+        V = np.dot(np.transpose(matrix), U / np.reshape(S, (1, -1)))
+    else:
+        S, V = np.linalg.eigh(np.dot(np.transpose(matrix), matrix))
+        S = np.sqrt(np.clip(S, np.finfo(S.dtype).eps, None))
+        U = np.dot(matrix, V) / np.reshape(S, (1, -1))
+    U, S, V = np.flip(U, axis=1), np.flip(S), np.flip(np.transpose(V), axis=0)
+    return U[:, : min(d1, n)], S[: min(d1, d2, n)], V[: min(d2, n), :]
+
+
+def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True,
+                  u_based_flip_sign=True, **kwargs):
+    """``tenalg/svd.py:366-447`` without mask imputation / non-negativity
+    (neither is on the path)."""
+    if method == "truncated_svd":
+        fun = truncated_svd
+    elif method == "symeig_svd":
+        fun = symeig_svd
+    elif callable(method):
+        fun = method
+    else:
+        raise ValueError(f"Got svd={method}. However, the possible choices are "
+                         "['truncated_svd', 'symeig_svd', 'randomized_svd'] or to pass a callable.")
+    U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs)
+    if flip_sign:
+        U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
+    return U, S, V
+
+
+# --------------------------------------------------------------------------
+# CP-ALS  (decomposition/_cp.py)
+# --------------------------------------------------------------------------
+def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=None,
+                  normalize_factors=False):
+    """``decomposition/_cp.py:26-137`` (no mask / non-negative branches)."""
+    rng = check_random_state(random_state)
+    if isinstance(init, str) and init == "random":
+        weights, factors = random_cp(tensor.shape, rank, normalise_factors=False,
+                                     random_state=rng, dtype=tensor.dtype)
+    elif isinstance(init, str) and init == "svd":
+        factors = []
+        for mode in range(tensor.ndim):
+            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd)
+            if mode == 0:
+                k = min(rank, S.shape[0])
+                U = U.copy()
+                U[:, :k] = U[:, :k] * S[:k]
+            if tensor.shape[mode] < rank:
+                pad = np.asarray(rng.random_sample((U.shape[0], rank - tensor.shape[mode])),
+                                 dtype=tensor.dtype)
+                U = np.concatenate([U, pad], axis=1)
+            factors.append(U[:, :rank])
+        weights = np.ones(rank, dtype=tensor.dtype)
+    elif isinstance(init, (tuple, list)):
+        weights, factors = init
+        factors = [np.array(f) for f in factors]
+        if weights is None or np.all(weights == 1):
+            return np.ones(rank, dtype=factors[0].dtype), factors
+        avg = np.prod(weights) ** (1.0 / weights.shape[0])
+        return np.ones(rank, dtype=factors[0].dtype), [f * avg for f in factors]
+    else:
+        raise ValueError(f'Initialization method "{init}" not recognized')
+    if normalize_factors:
+        weights, factors = cp_normalize((weights, factors))
+    return weights, factors
+
+
+def error_calc(tensor, norm_tensor, weights, factors, mttkrp=None):
+    """``decomposition/_cp.py:162-227`` (no mask / sparsity).
+
+    With the last mode's MTTKRP: ``sqrt(|‖X‖² + ‖M‖² − 2<X, M>|)`` where
+    ``<X, M> = sum(mttkrp * A_last)``; otherwise the full reconstruction."""
+    if mttkrp is None:
+        rec = cp_to_tensor((weights, factors))
+        return norm(tensor - rec, 2)
+    fn = cp_norm((weights, factors))
+    iprod = np.sum(np.sum(mttkrp * np.conj(factors[-1]), axis=0))
+    return np.sqrt(np.abs(norm_tensor ** 2 + fn ** 2 - 2 * iprod))
+
+
+def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
+            normalize_factors=False, orthogonalise=False, tol=1e-8, random_state=None,
+            verbose=0, return_errors=False, l2_reg=0, cvg_criterion="abs_rec_error",
+            fixed_modes=None, linesearch=False, callback=None, mttkrp_fn=None,
+            trace=None):
+    """``decomposition/_cp.py:230-543`` (no mask / sparsity).
+
+    ``mttkrp_fn`` swaps the MTTKRP (default: the materialised-KR form of
+    ``mttkrp.py:47-50``); ``trace``, if a list, receives per-iteration
+    records (line-search jump / accepted) for the tests."""
+    mttkrp_fn = mttkrp_fn or unfolding_dot_khatri_rao
+    rank = validate_cp_rank(tensor.shape, rank=rank)
+    if orthogonalise and not isinstance(orthogonalise, int):
+        orthogonalise = n_iter_max
+    if linesearch:
+        acc_pow, acc_fail, max_fail = 2.0, 0, 4
+    weights, factors = initialize_cp(tensor, rank, init=init, svd=svd,
+                                     random_state=random_state,
+                                     normalize_factors=normalize_factors)
+    factors = list(factors)
+    rec_errors = []
+    norm_tensor = norm(tensor, 2)
+    Id = np.eye(rank, dtype=tensor.dtype) * l2_reg if l2_reg else 0
+    fixed_modes = list(fixed_modes) if fixed_modes is not None else []
+    if fixed_modes == list(range(tensor.ndim)):
+        return (weights, factors)
+    if tensor.ndim - 1 in fixed_modes:
+        warnings.warn("You asked for fixing the last mode, which is not supported.")
+        fixed_modes.remove(tensor.ndim - 1)
+    modes_list = [m for m in range(tensor.ndim) if m not in fixed_modes]
+
+    if callback is not None:
+        err0 = error_calc(tensor, norm_tensor, weights, factors)
+        callback((weights, factors), err0 / norm_tensor)
+
+    rec_error = None
+    for iteration in range(n_iter_max):
+        if orthogonalise and iteration <= orthogonalise:
+            factors = [np.linalg.qr(f)[0] if min(f.shape) >= rank else f for f in factors]
+        if linesearch and iteration % 2 == 0:
+            factors_last = [f.copy() for f in factors]
+            weights_last = weights.copy()
+        for mode in modes_list:
+            V = np.ones((rank, rank), dtype=tensor.dtype)
+            for i, f in enumerate(factors):
+                if i != mode:
+                    V = V * np.dot(np.conj(np.transpose(f)), f)
+            V = V + Id
+            V = np.reshape(weights, (-1, 1)) * V * np.reshape(weights, (1, -1))
+            mttkrp = mttkrp_fn(tensor, (weights, factors), mode)
+            factors[mode] = np.transpose(np.linalg.solve(np.conj(np.transpose(V)),
+                                                         np.transpose(mttkrp)))
+        line_iter = bool(linesearch and iteration % 2 == 0 and iteration > 5)
+        if (tol or return_errors) and not line_iter:
+            unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
+        rec = {"iteration": iteration}
+        if line_iter:
+            jump = iteration ** (1.0 / acc_pow)
+            new_w = weights_last + (weights - weights_last) * jump
+            new_f = [factors_last[i] + (factors[i] - factors_last[i]) * jump
+                     for i in range(tensor.ndim)]
+            new_err = error_calc(tensor, norm_tensor, new_w, new_f)
+            rec["jump"] = jump
+            if new_err / norm_tensor < rec_errors[-1]:
+                factors, weights = new_f, new_w
+                unnorm = new_err
+                acc_fail = 0
+                rec["accepted"] = True
+            else:
+                unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
+                acc_fail += 1
+                rec["accepted"] = False
+                if acc_fail == max_fail:
+                    acc_pow += 1.0
+                    acc_fail = 0
+        if trace is not None:
+            trace.append(rec)
+        if (tol or return_errors) and not line_iter:
+            rec_error = unnorm / norm_tensor
+            rec_errors.append(rec_error)
+        if callback is not None:
+            if callback((weights, factors), rec_error) is True:
+                break
+        if tol:
+            if iteration >= 1:
+                dec = rec_errors[-2] - rec_errors[-1]
+                if cvg_criterion == "abs_rec_error":
+                    stop = abs(dec) < tol
+                elif cvg_criterion == "rec_error":
+                    stop = dec < tol
+                else:
+                    raise TypeError("Unknown convergence criterion")
+                if stop:
+                    break
+        if normalize_factors:
+            weights, factors = cp_normalize((weights, factors))
+    if return_errors:
+        return (weights, factors), rec_errors
+    return (weights, factors)
+
+
+# --------------------------------------------------------------------------
+# Tucker  (tucker_tensor.py, decomposition/_tucker.py)
+# --------------------------------------------------------------------------
+def validate_tucker_rank(tensor_shape, rank="same", rounding="round", fixed_modes=None):
+    """``tucker_tensor.py:344-447`` (int and explicit-tuple forms, and the
+    parameter-fraction form solved with brentq)."""
+    fun = {"ceil": np.ceil, "floor": np.floor, "round": np.round}.get(rounding)
+    if fun is None:
+        raise ValueError(f"Rounding should be round, floor or ceil, but got {rounding}")
+    if rank == "same":
+        rank = 1.0
+    if isinstance(rank, float):
+        from scipy.optimize import brentq
+        shape = list(tensor_shape)
+        n_modes = len(shape)
+        n_param = np.prod(shape)
+        fixed = []
+        n_fixed = 0
+        if fixed_modes is not None:
+            fixed = [(m, shape.pop(m)) for m in sorted(fixed_modes, reverse=True)][::-1]
+            n_fixed = np.sum([s ** 2 for _, s in fixed])
+            n_modes -= len(fixed)
+        sq = np.sum([s ** 2 for s in shape])
+        frac = brentq(lambda x: n_param * x ** n_modes + sq * x + n_fixed * x - rank * n_param,
+                      0.0, max(rank, 1.0))
+        out = [max(int(fun(s * frac)), 1) for s in shape]
+        for m, s in fixed:
+            out.insert(m, s)
+        return out
+    if isinstance(rank, int):
+        if fixed_modes is None:
+            return [rank] * len(tensor_shape)
+        return [rank if i not in fixed_modes else s for i, s in enumerate(tensor_shape)]
+    return rank
+
+
+def tucker_to_tensor(tucker_tensor, skip_factor=None, transpose_factors=False, modes=None):
+    """``tucker_tensor.py:50-75``."""
+    core, factors = tucker_tensor
+    return multi_mode_dot(core, factors, skip=skip_factor, transpose=transpose_factors,
+                          modes=modes)
+
+
+def initialize_tucker(tensor, rank, modes, random_state, init="svd", svd="truncated_svd"):
+    """``decomposition/_tucker.py:25-102`` (no mask / non-negative)."""
+    if isinstance(init, str) and init == "svd":
+        factors = []
+        for i, mode in enumerate(modes):
+            U, _, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank[i], method=svd)
+            factors.append(U)
+        core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
+    elif isinstance(init, str) and init == "random":
+        rng = check_random_state(random_state)
+        core = np.asarray(rng.random_sample([rank[i] for i in range(len(modes))]) + 0.01,
+                          dtype=tensor.dtype)
+        factors = [np.asarray(rng.random_sample((tensor.shape[m], rank[i])), dtype=tensor.dtype)
+                   for i, m in enumerate(modes)]
+    else:
+        core, factors = init
+        factors = list(factors)
+    return core, factors
+
+
+def partial_tucker(tensor, rank, modes=None, n_iter_max=100, init="svd", tol=10e-5,
+                   svd="truncated_svd", random_state=None, verbose=False):
+    """``decomposition/_tucker.py:105-221``: HOOI.  Inside the loop the factor
+    update always uses the default ``truncated_svd`` (``:197-202``)."""
+    if modes is None:
+        modes = list(range(tensor.ndim))
+    if rank is None:
+        rank = [tensor.shape[m] for m in modes]
+    elif isinstance(rank, int):
+        rank = tuple(rank for _ in modes)
+    else:
+        rank = tuple(rank)
+    core, factors = initialize_tucker(tensor, rank, modes, init=init, svd=svd,
+                                      random_state=random_state)
+    rec_errors = []
+    norm_tensor = norm(tensor, 2)
+    for iteration in range(n_iter_max):
+        for index, mode in enumerate(modes):
+            y = multi_mode_dot(tensor, factors, modes=modes, skip=index, transpose=True)
+            U, _, _ = svd_interface(unfold(y, mode), n_eigenvecs=rank[index])
+            factors[index] = U
+        core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
+        rec_errors.append(math.sqrt(abs(norm_tensor ** 2 - norm(core, 2) ** 2)) / norm_tensor)
+        if iteration > 1:
+            if tol and abs(rec_errors[-2] - rec_errors[-1]) < tol:
+                break
+    return (core, factors), rec_errors
+
+
+def tucker(tensor, rank, fixed_factors=None, n_iter_max=100, init="svd", return_errors=False,
+           svd="truncated_svd", tol=10e-5, random_state=None, verbose=False):
+    """``decomposition/_tucker.py:224-345``."""
+    if fixed_factors:
+        core, factors = init
+        fixed_factors = sorted(fixed_factors)
+        fixed = [(i, f) for i, f in enumerate(factors) if i in fixed_factors]
+        modes_fixed = [i for i, _ in fixed]
+        factors_fixed = [f for _, f in fixed]
+        core = multi_mode_dot(core, factors_fixed, modes=modes_fixed)
+        free = [(i, f) for i, f in enumerate(factors) if i not in fixed_factors]
+        modes = [i for i, _ in free]
+        (core, new_factors), errs = partial_tucker(
+            tensor, rank=rank, modes=modes, n_iter_max=n_iter_max,
+            init=(core, [f for _, f in free]), svd=svd, tol=tol, random_state=random_state)
+        factors = list(new_factors)
+        for i, e in enumerate(fixed_factors):
+            factors.insert(e, factors_fixed[i])
+        core = multi_mode_dot(core, factors_fixed, modes=modes_fixed, transpose=True)
+        return (core, factors)
+    modes = list(range(tensor.ndim))
+    rank = validate_tucker_rank(tensor.shape, rank=rank)
+    (core, factors), errs = partial_tucker(tensor, rank=rank, modes=modes,
+                                           n_iter_max=n_iter_max, init=init, svd=svd,
+                                           tol=tol, random_state=random_state)
+    if return_errors:
+        return (core, factors), errs
+    return (core, factors)

@@ -1,0 +1,77 @@
+// Shared helpers for the gfx950 kernels of the CP-ALS / Tucker-HOOI path.
+//
+// Conventions used by every kernel in this library:
+//  * dense tensors are C-order (last mode contiguous), factors are row-major
+//    (I_m x R) -- the layouts tensorly's numpy/pytorch backends hand over;
+//  * every entry point returns an int status (0 = ok) and never allocates:
+//    scratch memory ("workspace") is passed in by the caller, sized by the
+//    matching *_workspace_size() query;
+//  * reductions are two-stage with a fixed order, so repeated calls are
+//    bitwise identical (no floating-point atomics anywhere).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/tensorly_amd.h"
+
+namespace tla {
+
+void set_error(const char* fmt, ...);
+
+#define TLA_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      ::tla::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,                  \
+                       hipGetErrorString(_e));                                     \
+      return TLA_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+#define TLA_LAUNCH_CHECK() TLA_HIP(hipGetLastError())
+
+#define TLA_REQUIRE(cond, ...)                                                     \
+  do {                                                                             \
+    if (!(cond)) {                                                                 \
+      ::tla::set_error(__VA_ARGS__);                                               \
+      return TLA_ERR_ARG;                                                          \
+    }                                                                              \
+  } while (0)
+
+constexpr int kMaxOrder = 8;
+constexpr int kWave = 64;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma_f32_16x16x4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma_bf16_16x16x32(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Round-to-nearest-even split x = hi + lo (+ O(2^-16 |x|)).
+__device__ __forceinline__ void split_bf16(const floatx8& x, bf16x8& hi, bf16x8& lo) {
+  hi = __builtin_convertvector(x, bf16x8);
+  floatx8 back = __builtin_convertvector(hi, floatx8);
+  lo = __builtin_convertvector(x - back, bf16x8);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+struct ModeShape {
+  int ndim;
+  int64_t shape[kMaxOrder];
+};
+
+}  // namespace tla

@@ -1,0 +1,351 @@
+// Native CP-ALS sweep driver: the mode loop of
+// tensorly/decomposition/_cp.py:394-531 issued from C++ so that one host call
+// runs any number of sweeps with no host synchronisation.
+//
+// Per sweep and per free mode n:
+//   M = MTTKRP(X, factors, weights, n)          (fused kernel, one tensor pass)
+//   factors[n] = M V^-1, V = (*_{m != n} G_m + l2 I) * w w^T   (FP64 Cholesky)
+//   G_n = factors[n]^T factors[n]                (FP64, fused into the solve)
+// and after the last mode (whose MTTKRP is kept) the error of
+// _cp.py:217-225 from the Grams: sqrt(|‖X‖² + ‖M‖² − 2<X, M>|) / ‖X‖.
+//
+// Dimension tree (orders 3 and 4): the first two modes share
+// T = X contracted with the factors of modes >= 2 (one pass), the remaining
+// modes share the contraction of X with the *updated* first two factors (one
+// more pass), so a sweep reads the tensor twice instead of N times.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace tla {
+
+struct GramPtrs {
+  const double* p[kMaxOrder];
+};
+
+__global__ void normalize_scales_kernel(GramPtrs grams, int ndim, int R, void* weights,
+                                        double* scales /* ndim x R */) {
+  // weights fold into factor 0 (tensorly/cp_tensor.py:278-321); the column
+  // norms come from the FP64 Gram diagonals.
+  const int r = threadIdx.x;
+  if (r >= R) return;
+  float* w = (float*)weights;
+  const double w0 = (double)w[r];
+  double neww = 1.0;
+  for (int m = 0; m < ndim; ++m) {
+    double s = sqrt(fmax(grams.p[m][r * R + r], 0.0));
+    if (m == 0) s *= fabs(w0);
+    neww *= s;
+    // factor m column r is multiplied by f: for m == 0 that is w0 / s.
+    const double f = (s == 0.0) ? ((m == 0) ? w0 : 1.0) : ((m == 0) ? w0 / s : 1.0 / s);
+    scales[m * R + r] = f;
+  }
+  w[r] = (float)neww;
+}
+
+__global__ void scale_columns_kernel(float* A, int64_t rows, int R, const double* f) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * R) return;
+  A[e] = (float)((double)A[e] * f[e % R]);
+}
+
+struct CpAls {
+  int x_dtype, ndim, R, precision, dimtree;
+  const void* X;
+  int64_t shape[kMaxOrder];
+  void* factors[kMaxOrder];
+  void* weights;
+  double l2;
+  bool fixed[kMaxOrder];
+  // workspace carve-up
+  double* grams[kMaxOrder];
+  double* normx2;
+  double* iprod;
+  double* scales;
+  int* status;
+  float* M;
+  float* T1;
+  float* T2;
+  void* ws;  // shared scratch for the kernels' own workspaces
+  size_t ws_bytes;
+  bool initialised;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Layout {
+  size_t grams, normx2, iprod, scales, status, M, T1, T2, ws, total;
+  size_t ws_bytes;
+};
+
+static bool dimtree_ok(int x_dtype, int ndim, const int64_t* shape, int R, int precision) {
+  if (!(ndim == 3 || ndim == 4)) return false;
+  if (x_dtype == TLA_F64 || precision == TLA_PREC_VALU) return false;
+  if (tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 0, 2, precision) == 0) return false;
+  if (ndim == 4 && tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 2, 4, precision) == 0)
+    return false;
+  return true;
+}
+
+static Layout layout(int x_dtype, int ndim, const int64_t* shape, int R, int precision, int dimtree) {
+  Layout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  const size_t esz = (x_dtype == TLA_F64) ? 8 : 4;
+  L.grams = take(size_t(ndim) * R * R * 8);
+  L.normx2 = take(8);
+  L.iprod = take(8);
+  L.scales = take(size_t(ndim) * R * 8);
+  L.status = take(8);
+  int64_t maxI = 0;
+  int64_t total = 1;
+  for (int m = 0; m < ndim; ++m) { maxI = std::max(maxI, shape[m]); total *= shape[m]; }
+  L.M = take(size_t(maxI) * R * esz);
+  size_t ws = tla_sqnorm_workspace_size(total);
+  for (int m = 0; m < ndim; ++m) {
+    ws = std::max(ws, tla_mttkrp_workspace_size(x_dtype, ndim, shape, R, m, precision));
+    ws = std::max(ws, tla_cp_solve_workspace_size(shape[m], R));
+    ws = std::max(ws, tla_gram64_workspace_size(shape[m], R));
+  }
+  if (dimtree) {
+    L.T1 = take(size_t(shape[0]) * shape[1] * R * 4);
+    ws = std::max(ws, tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 0, 2, precision));
+    ws = std::max(ws, tla_rankdiag_contract_workspace_size(shape[0], shape[1], R, 0));
+    if (ndim == 4) {
+      L.T2 = take(size_t(shape[2]) * shape[3] * R * 4);
+      ws = std::max(ws, tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 2, 4, precision));
+      ws = std::max(ws, tla_rankdiag_contract_workspace_size(shape[2], shape[3], R, 0));
+    }
+  }
+  L.ws = take(ws);
+  L.ws_bytes = ws;
+  L.total = off;
+  return L;
+}
+
+static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, hipStream_t st) {
+  const double* gp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
+  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  return tla_cp_solve(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n],
+                      h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->ws, h->ws_bytes, st);
+}
+
+static int mttkrp_mode(CpAls* h, int n, hipStream_t st) {
+  const void* fp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) fp[m] = h->factors[m];
+  return tla_mttkrp(h->x_dtype, h->X, h->ndim, h->shape, fp, h->weights, h->R, n, h->precision, h->M,
+                    h->ws, h->ws_bytes, st);
+}
+
+static int sweep_standard(CpAls* h, hipStream_t st) {
+  const int N = h->ndim;
+  for (int n = 0; n < N; ++n) {
+    if (h->fixed[n]) continue;
+    int rc = mttkrp_mode(h, n, st);
+    if (rc) return rc;
+    rc = solve_mode(h, n, h->M, n == N - 1, st);
+    if (rc) return rc;
+  }
+  return TLA_OK;
+}
+
+static int sweep_dimtree(CpAls* h, hipStream_t st) {
+  const int N = h->ndim;
+  const void* fp[kMaxOrder];
+  const int R = h->R;
+  // First half: modes 0 and 1 share T1 = X x_{m>=2} A_m.
+  if (!(h->fixed[0] && h->fixed[1])) {
+    for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
+    int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 0, 2, h->precision, h->T1,
+                                h->ws, h->ws_bytes, st);
+    if (rc) return rc;
+    if (!h->fixed[0]) {
+      rc = tla_rankdiag_contract(h->T1, h->shape[0], h->shape[1], R, (const float*)h->factors[1], 1, h->M,
+                                 h->ws, h->ws_bytes, st);
+      if (rc) return rc;
+      rc = solve_mode(h, 0, h->M, false, st);
+      if (rc) return rc;
+    }
+    if (!h->fixed[1]) {
+      rc = tla_rankdiag_contract(h->T1, h->shape[0], h->shape[1], R, (const float*)h->factors[0], 0, h->M,
+                                 h->ws, h->ws_bytes, st);
+      if (rc) return rc;
+      rc = solve_mode(h, 1, h->M, false, st);
+      if (rc) return rc;
+    }
+  }
+  if (N == 3) {
+    int rc = mttkrp_mode(h, 2, st);
+    if (rc) return rc;
+    return solve_mode(h, 2, h->M, true, st);
+  }
+  // N == 4: modes 2 and 3 share T2 = X x_0 A_0 x_1 A_1 (updated factors).
+  for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
+  int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 2, 4, h->precision, h->T2,
+                              h->ws, h->ws_bytes, st);
+  if (rc) return rc;
+  if (!h->fixed[2]) {
+    rc = tla_rankdiag_contract(h->T2, h->shape[2], h->shape[3], R, (const float*)h->factors[3], 1, h->M, h->ws,
+                               h->ws_bytes, st);
+    if (rc) return rc;
+    rc = solve_mode(h, 2, h->M, false, st);
+    if (rc) return rc;
+  }
+  rc = tla_rankdiag_contract(h->T2, h->shape[2], h->shape[3], R, (const float*)h->factors[2], 0, h->M, h->ws,
+                             h->ws_bytes, st);
+  if (rc) return rc;
+  return solve_mode(h, 3, h->M, true, st);
+}
+
+}  // namespace tla
+
+using namespace tla;
+
+extern "C" size_t tla_cp_als_workspace_size(int x_dtype, int ndim, const int64_t* shape, int rank, int precision,
+                                            int dimtree) {
+  if (ndim < 2 || ndim > kMaxOrder || rank < 1 || rank > 64) return 0;
+  if (x_dtype == TLA_F64) precision = TLA_PREC_VALU;
+  if (dimtree && !dimtree_ok(x_dtype, ndim, shape, rank, precision)) dimtree = 0;
+  return layout(x_dtype, ndim, shape, rank, precision, dimtree).total;
+}
+
+extern "C" int tla_cp_als_dimtree_supported(int x_dtype, int ndim, const int64_t* shape, int rank,
+                                            int precision) {
+  return dimtree_ok(x_dtype, ndim, shape, rank, precision) ? 1 : 0;
+}
+
+extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int64_t* shape, int rank,
+                                 void* const* factors, void* weights, double l2_reg, int precision,
+                                 int dimtree, const int* fixed_modes, void* workspace, size_t ws_bytes,
+                                 void** handle_out) {
+  TLA_REQUIRE(x_dtype == TLA_F32 || x_dtype == TLA_F64 || x_dtype == TLA_BF16, "cp_als: bad dtype");
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_als: order %d", ndim);
+  TLA_REQUIRE(rank >= 1 && rank <= 64, "cp_als: rank %d not in [1, 64]", rank);
+  TLA_REQUIRE(X && factors && weights && workspace && handle_out, "cp_als: NULL pointer");
+  if (x_dtype == TLA_F64) precision = TLA_PREC_VALU;
+  TLA_REQUIRE(x_dtype != TLA_BF16 || precision == TLA_PREC_BF16, "cp_als: bf16 tensor needs bf16 precision");
+  if (dimtree && !dimtree_ok(x_dtype, ndim, shape, rank, precision)) dimtree = 0;
+  const Layout L = layout(x_dtype, ndim, shape, rank, precision, dimtree);
+  TLA_REQUIRE(ws_bytes >= L.total, "cp_als: workspace too small (%zu < %zu)", ws_bytes, L.total);
+  CpAls* h = new CpAls{};
+  h->x_dtype = x_dtype; h->ndim = ndim; h->R = rank; h->precision = precision; h->dimtree = dimtree;
+  h->X = X; h->weights = weights; h->l2 = l2_reg;
+  char* base = (char*)workspace;
+  for (int m = 0; m < ndim; ++m) {
+    h->shape[m] = shape[m];
+    h->factors[m] = factors[m];
+    h->fixed[m] = fixed_modes ? fixed_modes[m] != 0 : false;
+    h->grams[m] = (double*)(base + L.grams) + size_t(m) * rank * rank;
+  }
+  h->fixed[ndim - 1] = false;  // the last mode is never fixed (_cp.py:365-369)
+  h->normx2 = (double*)(base + L.normx2);
+  h->iprod = (double*)(base + L.iprod);
+  h->scales = (double*)(base + L.scales);
+  h->status = (int*)(base + L.status);
+  h->M = (float*)(base + L.M);
+  h->T1 = dimtree ? (float*)(base + L.T1) : nullptr;
+  h->T2 = (dimtree && ndim == 4) ? (float*)(base + L.T2) : nullptr;
+  h->ws = base + L.ws;
+  h->ws_bytes = L.ws_bytes;
+  h->initialised = false;
+  *handle_out = h;
+  return TLA_OK;
+}
+
+extern "C" int tla_cp_als_dimtree_active(void* handle) { return handle ? ((CpAls*)handle)->dimtree : 0; }
+
+extern "C" void tla_cp_als_destroy(void* handle) { delete (CpAls*)handle; }
+
+// (Re)compute ‖X‖² and the Grams of the current factors (after the caller
+// changed the factors, e.g. a line-search step).
+extern "C" int tla_cp_als_refresh(void* handle, int with_norm, void* stream) {
+  CpAls* h = (CpAls*)handle;
+  TLA_REQUIRE(h, "cp_als: NULL handle");
+  hipStream_t st = (hipStream_t)stream;
+  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  if (with_norm) {
+    int64_t total = 1;
+    for (int m = 0; m < h->ndim; ++m) total *= h->shape[m];
+    TLA_REQUIRE(h->x_dtype != TLA_BF16, "cp_als: norm of a bf16 copy is not computed here");
+    int rc = tla_sqnorm(dt, h->X, total, h->normx2, h->ws, h->ws_bytes, st);
+    if (rc) return rc;
+  }
+  for (int m = 0; m < h->ndim; ++m) {
+    int rc = tla_gram64(dt, h->factors[m], h->shape[m], h->R, h->grams[m], h->ws, h->ws_bytes, st);
+    if (rc) return rc;
+  }
+  h->initialised = true;
+  return TLA_OK;
+}
+
+extern "C" int tla_cp_als_set_normx2(void* handle, const double* normx2_device, void* stream) {
+  CpAls* h = (CpAls*)handle;
+  TLA_REQUIRE(h && normx2_device, "cp_als: NULL pointer");
+  TLA_HIP(hipMemcpyAsync(h->normx2, normx2_device, 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return TLA_OK;
+}
+
+// Run n sweeps; errors[first_slot + s] receives the relative error after
+// sweep s (if errors != NULL).  normalize: cp_normalize after each sweep.
+extern "C" int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int first_slot, int normalize,
+                                 void* stream) {
+  CpAls* h = (CpAls*)handle;
+  TLA_REQUIRE(h, "cp_als: NULL handle");
+  TLA_REQUIRE(h->initialised, "cp_als: call tla_cp_als_refresh first");
+  hipStream_t st = (hipStream_t)stream;
+  const double* gp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
+  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  for (int s = 0; s < n_sweeps; ++s) {
+    int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
+    if (rc) return rc;
+    if (errors) {
+      rc = tla_cp_error(gp, h->ndim, h->weights, dt, h->R, h->normx2, h->iprod, errors, first_slot + s, st);
+      if (rc) return rc;
+    }
+    if (normalize) {
+      TLA_REQUIRE(dt == TLA_F32, "cp_als: normalize_factors supports float32");
+      GramPtrs g{};
+      for (int m = 0; m < h->ndim; ++m) g.p[m] = h->grams[m];
+      normalize_scales_kernel<<<1, 64, 0, st>>>(g, h->ndim, h->R, h->weights, h->scales);
+      TLA_LAUNCH_CHECK();
+      for (int m = 0; m < h->ndim; ++m) {
+        const int64_t ne = h->shape[m] * h->R;
+        scale_columns_kernel<<<ceil_div(ne, 256), 256, 0, st>>>((float*)h->factors[m], h->shape[m], h->R,
+                                                                h->scales + m * h->R);
+        TLA_LAUNCH_CHECK();
+        rc = tla_gram64(dt, h->factors[m], h->shape[m], h->R, h->grams[m], h->ws, h->ws_bytes, st);
+        if (rc) return rc;
+      }
+    }
+  }
+  return TLA_OK;
+}
+
+// Copy part of the driver state to a device buffer (stream-ordered):
+// 0 grams (ndim x R x R FP64), 1 ‖X‖² (FP64), 2 last <M, A> (FP64),
+// 3 solve status (int).
+extern "C" int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t bytes, void* stream) {
+  CpAls* h = (CpAls*)handle;
+  TLA_REQUIRE(h && dst, "cp_als: NULL pointer");
+  const void* src = nullptr;
+  size_t avail = 0;
+  switch (which) {
+    case 0: src = h->grams[0]; avail = size_t(h->ndim) * h->R * h->R * 8; break;
+    case 1: src = h->normx2; avail = 8; break;
+    case 2: src = h->iprod; avail = 8; break;
+    case 3: src = h->status; avail = 4; break;
+    default: TLA_REQUIRE(false, "cp_als: unknown state %d", which);
+  }
+  TLA_REQUIRE(bytes <= avail, "cp_als: %zu bytes requested, state %d holds %zu", bytes, which, avail);
+  TLA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return TLA_OK;
+}

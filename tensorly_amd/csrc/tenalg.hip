@@ -1,0 +1,462 @@
+// Khatri-Rao product, n-mode product (TTM / TTV) and the rank-diagonal
+// contraction used by the dimension-tree MTTKRP.
+//
+// mode_dot has two MFMA kernels for the Tucker chain (FP32 MFMA, BF16 or
+// split BF16x3, FP32 accumulation):
+//  * ttm_rows   -- contracting the contiguous (last) mode:
+//                  out[p, j] = sum_k X[p, k] U'[j, k], 16 rows per wave, the
+//                  U' panel staged transposed in LDS;
+//  * ttm_cols   -- contracting an inner mode: out[a, j, b] =
+//                  sum_m U'[j, m] X[a, m, b] with X as the MFMA B operand
+//                  (16-byte loads of 4 consecutive b per lane, 256 bytes per
+//                  row per wave) and the m-range split over the 4 waves of a
+//                  block, reduced in LDS in a fixed order.
+// and a vector-ALU kernel for everything else (vectors, f64, J > 64, odd
+// sizes).
+#include <algorithm>
+
+#include "common.h"
+
+namespace tla {
+
+// ---------------------------------------------------------------------------
+// Khatri-Rao
+// ---------------------------------------------------------------------------
+struct KrArgs {
+  int n, R;
+  int64_t rows[kMaxOrder];
+  const void* mats[kMaxOrder];
+  const void* weights;
+  const void* mask;
+  int64_t total;
+};
+
+template <typename T>
+__global__ void khatri_rao_kernel(KrArgs a, T* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.total * a.R) return;
+  const int r = (int)(e % a.R);
+  int64_t q = e / a.R;
+  const int64_t q0 = q;
+  const T w = a.weights ? ((const T*)a.weights)[r] : (T)1;
+  int64_t idx[kMaxOrder];
+#pragma unroll
+  for (int m = kMaxOrder - 1; m >= 0; --m) {
+    if (m < a.n) {
+      idx[m] = q % a.rows[m];
+      q /= a.rows[m];
+    }
+  }
+  // (w * A_0) (x) A_1 (x) ... in that rounding order
+  T res = ((const T*)a.mats[0])[idx[0] * a.R + r] * w;
+#pragma unroll
+  for (int m = 1; m < kMaxOrder; ++m)
+    if (m < a.n) res = res * ((const T*)a.mats[m])[idx[m] * a.R + r];
+  if (a.mask) res = res * ((const T*)a.mask)[q0];
+  out[e] = res;
+}
+
+// ---------------------------------------------------------------------------
+// mode_dot, vector-ALU: out[a, j, b] = sum_m Uj(j, m) X[a, m, b]
+// Uj(j, m) = U[j * ldj + m * ldm]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void ttm_generic_kernel(const T* __restrict__ X, int64_t Pa, int64_t Im, int64_t Pb,
+                                   const T* __restrict__ U, int64_t J, int64_t ldj, int64_t ldm,
+                                   T* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Pa * J * Pb) return;
+  const int64_t b = e % Pb;
+  const int64_t t = e / Pb;
+  const int64_t j = t % J, a = t / J;
+  const T* x = X + a * Im * Pb + b;
+  const T* u = U + j * ldj;
+  T acc = 0;
+  for (int64_t m = 0; m < Im; ++m) acc += u[m * ldm] * x[m * Pb];
+  out[e] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// ttm_rows: out[p, j] = sum_k X[p, k] Ut[k, j]   (Ut = K x J, row-major view
+// given by ld strides), J <= 64.
+// ---------------------------------------------------------------------------
+struct TtmRowsArgs {
+  const float* X;
+  int64_t P, K;
+  const float* U;
+  int64_t ldj, ldk;  // U'(j, k) = U[j * ldj + k * ldk]
+  int J;
+  float* out;
+  int panel;
+};
+
+template <int PREC, int RT>
+__global__ __launch_bounds__(256) void ttm_rows_kernel(TtmRowsArgs a) {
+  constexpr int RP = RT * 16;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int LDF = a.panel + 4, LDB = a.panel + 8;
+  float* Ct = (float*)smem;
+  __bf16* Ch = (__bf16*)smem;
+  __bf16* Cl = Ch + RP * LDB;
+  const int64_t p = ((int64_t)blockIdx.x * 4 + wave) * 16 + c;
+  const int64_t rp = p < a.P ? p : a.P - 1;
+  const float* row = a.X + rp * a.K + 8 * g;
+  floatx4 d[RT];
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) d[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t kb = 0; kb < a.K; kb += a.panel) {
+    const int klen = (int)min((int64_t)a.panel, a.K - kb);
+    __syncthreads();
+    for (int idx = tid; idx < RP * klen; idx += 256) {
+      const int n = idx % RP, kk = idx / RP;
+      const float v = (n < a.J) ? a.U[n * a.ldj + (kb + kk) * a.ldk] : 0.0f;
+      if constexpr (PREC == TLA_PREC_FP32) {
+        Ct[n * LDF + kk] = v;
+      } else {
+        const __bf16 h = (__bf16)v;
+        Ch[n * LDB + kk] = h;
+        if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+      }
+    }
+    __syncthreads();
+    for (int kk = 0; kk < klen; kk += 32) {
+      const floatx4 x0 = *(const floatx4*)(row + kb + kk);
+      const floatx4 x1 = *(const floatx4*)(row + kb + kk + 4);
+      floatx8 av;
+      av[0] = x0[0]; av[1] = x0[1]; av[2] = x0[2]; av[3] = x0[3];
+      av[4] = x1[0]; av[5] = x1[1]; av[6] = x1[2]; av[7] = x1[3];
+      if constexpr (PREC == TLA_PREC_FP32) {
+#pragma unroll
+        for (int nb = 0; nb < RT; ++nb) {
+          const float* bp = Ct + (nb * 16 + c) * LDF + kk + 8 * g;
+          const floatx4 b0 = *(const floatx4*)bp;
+          const floatx4 b1 = *(const floatx4*)(bp + 4);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) d[nb] = mfma_f32_16x16x4(av[v], b0[v], d[nb]);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) d[nb] = mfma_f32_16x16x4(av[4 + v], b1[v], d[nb]);
+        }
+      } else {
+        bf16x8 ah, al;
+        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(av, ah, al);
+        else ah = __builtin_convertvector(av, bf16x8);
+#pragma unroll
+        for (int nb = 0; nb < RT; ++nb) {
+          const int off = (nb * 16 + c) * LDB + kk + 8 * g;
+          const bf16x8 bh = *(const bf16x8*)(Ch + off);
+          d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) {
+            const bf16x8 bl = *(const bf16x8*)(Cl + off);
+            d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+            d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
+          }
+        }
+      }
+    }
+  }
+  // d[nb][e] = out[tile row 4g + e][nb*16 + c]
+  const int64_t tile0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) {
+    const int j = nb * 16 + c;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t pr = tile0 + 4 * g + e;
+      if (pr < a.P && j < a.J) a.out[pr * a.J + j] = d[nb][e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ttm_cols: out[a, j, b] = sum_m U'(j, m) X[a, m, b], J <= 64, Pb % 4 == 0.
+// grid.x = 64-column tiles of b, grid.y = a; 4 waves split the m range.
+// ---------------------------------------------------------------------------
+struct TtmColsArgs {
+  const float* X;
+  int64_t Pa, Im, Pb;
+  const float* U;
+  int64_t ldj, ldm;
+  int J;
+  float* out;
+};
+
+template <int PREC, int RT>
+__global__ __launch_bounds__(256) void ttm_cols_kernel(TtmColsArgs a) {
+  constexpr int RP = RT * 16;
+  constexpr int STEP = (PREC == TLA_PREC_FP32) ? 4 : 32;
+  __shared__ float red[64 * RP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int64_t col = (int64_t)blockIdx.x * 64 + 4 * c;
+  const bool colok = col < a.Pb;
+  const int64_t ia = blockIdx.y;
+  const float* Xa = a.X + ia * a.Im * a.Pb;
+  // m-range of this wave (multiple of STEP except the end)
+  const int64_t per = ((a.Im + 4 * STEP - 1) / (4 * STEP)) * STEP;
+  const int64_t mb0 = wave * per;
+  const int64_t me = min(a.Im, mb0 + per);
+  floatx4 acc[4][RT];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int mb = 0; mb < RT; ++mb) acc[t][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t m0 = mb0; m0 < me; m0 += STEP) {
+    if constexpr (PREC == TLA_PREC_FP32) {
+      const int64_t m = m0 + g;
+      const bool ok = m < me;
+      floatx4 xv = floatx4{0.f, 0.f, 0.f, 0.f};
+      if (ok && colok) xv = *(const floatx4*)(Xa + m * a.Pb + col);
+      float uv[RT];
+#pragma unroll
+      for (int mb = 0; mb < RT; ++mb) {
+        const int j = mb * 16 + c;
+        uv[mb] = (ok && j < a.J) ? a.U[j * a.ldj + m * a.ldm] : 0.0f;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mb = 0; mb < RT; ++mb) acc[t][mb] = mfma_f32_16x16x4(uv[mb], xv[t], acc[t][mb]);
+    } else {
+      floatx4 xv[8];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int64_t m = m0 + 8 * g + v;
+        xv[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (m < me && colok) xv[v] = *(const floatx4*)(Xa + m * a.Pb + col);
+      }
+      bf16x8 uh[RT], ul[RT];
+#pragma unroll
+      for (int mb = 0; mb < RT; ++mb) {
+        const int j = mb * 16 + c;
+        floatx8 u8;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int64_t m = m0 + 8 * g + v;
+          u8[v] = (m < me && j < a.J) ? a.U[j * a.ldj + m * a.ldm] : 0.0f;
+        }
+        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(u8, uh[mb], ul[mb]);
+        else uh[mb] = __builtin_convertvector(u8, bf16x8);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        floatx8 x8;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) x8[v] = xv[v][t];
+        bf16x8 xh, xl;
+        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(x8, xh, xl);
+        else xh = __builtin_convertvector(x8, bf16x8);
+#pragma unroll
+        for (int mb = 0; mb < RT; ++mb) {
+          acc[t][mb] = mfma_bf16_16x16x32(uh[mb], xh, acc[t][mb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) {
+            acc[t][mb] = mfma_bf16_16x16x32(uh[mb], xl, acc[t][mb]);
+            acc[t][mb] = mfma_bf16_16x16x32(ul[mb], xh, acc[t][mb]);
+          }
+        }
+      }
+    }
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mb = 0; mb < RT; ++mb)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int idx = (4 * c + t) * RP + mb * 16 + 4 * g + e;
+            red[idx] = (w == 0 ? 0.0f : red[idx]) + acc[t][mb][e];
+          }
+    }
+    __syncthreads();
+  }
+  // out[a, j, b]: b fastest -- write 64 consecutive b for each j
+  for (int idx = tid; idx < 64 * RP; idx += 256) {
+    const int j = idx / 64, bc = idx % 64;
+    const int64_t b = (int64_t)blockIdx.x * 64 + bc;
+    if (b < a.Pb && j < a.J) a.out[(ia * a.J + j) * a.Pb + b] = red[bc * RP + j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rank-diagonal contraction of Y (I x J x R)
+// ---------------------------------------------------------------------------
+// axis 1: out[i, r] = sum_j Y[i, j, r] F[j, r]  -- one block per i
+__global__ __launch_bounds__(256) void rankdiag_axis1_kernel(const float* __restrict__ Y, int64_t J, int R,
+                                                             const float* __restrict__ F,
+                                                             float* __restrict__ out) {
+  const int64_t i = blockIdx.x;
+  __shared__ float red[256];
+  // thread t handles r = t % R over j = t / R, t / R + (256 / R), ...
+  const int lanes_per_r = 256 / R;
+  const int r = threadIdx.x % R, js = threadIdx.x / R;
+  float s = 0.0f;
+  if (js < lanes_per_r)
+    for (int64_t j = js; j < J; j += lanes_per_r) s += Y[(i * J + j) * R + r] * F[j * R + r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    float t = 0.0f;
+    for (int k = 0; k < lanes_per_r; ++k) t += red[k * R + threadIdx.x];
+    out[i * R + threadIdx.x] = t;
+  }
+}
+
+// axis 0: out[j, r] = sum_i Y[i, j, r] F[i, r] -- partials over i-chunks
+__global__ __launch_bounds__(256) void rankdiag_axis0_kernel(const float* __restrict__ Y, int64_t I, int64_t J,
+                                                             int R, const float* __restrict__ F,
+                                                             int64_t i_per, float* __restrict__ partial) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over J*R
+  if (e >= J * R) return;
+  const int r = (int)(e % R);
+  const int64_t i0 = (int64_t)blockIdx.y * i_per;
+  const int64_t i1 = min(I, i0 + i_per);
+  float s = 0.0f;
+  for (int64_t i = i0; i < i1; ++i) s += Y[i * J * R + e] * F[i * R + r];
+  partial[(int64_t)blockIdx.y * J * R + e] = s;
+}
+
+__global__ void sum_splits_kernel(const float* __restrict__ partial, int64_t n, int splits,
+                                  float* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += partial[(int64_t)k * n + e];
+  out[e] = s;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <int PREC>
+static int launch_ttm_rows(const TtmRowsArgs& a, hipStream_t st) {
+  const int RT = (a.J + 15) / 16;
+  const size_t smem = (PREC == TLA_PREC_FP32) ? size_t(RT * 16) * (a.panel + 4) * 4
+                                              : size_t(2) * RT * 16 * (a.panel + 8) * 2;
+  const dim3 grid((unsigned)ceil_div(a.P, 64));
+  switch (RT) {
+    case 1: ttm_rows_kernel<PREC, 1><<<grid, 256, smem, st>>>(a); break;
+    case 2: ttm_rows_kernel<PREC, 2><<<grid, 256, smem, st>>>(a); break;
+    case 3: ttm_rows_kernel<PREC, 3><<<grid, 256, smem, st>>>(a); break;
+    default: ttm_rows_kernel<PREC, 4><<<grid, 256, smem, st>>>(a); break;
+  }
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+template <int PREC>
+static int launch_ttm_cols(const TtmColsArgs& a, hipStream_t st) {
+  const int RT = (a.J + 15) / 16;
+  const dim3 grid((unsigned)ceil_div(a.Pb, 64), (unsigned)a.Pa);
+  switch (RT) {
+    case 1: ttm_cols_kernel<PREC, 1><<<grid, 256, 0, st>>>(a); break;
+    case 2: ttm_cols_kernel<PREC, 2><<<grid, 256, 0, st>>>(a); break;
+    case 3: ttm_cols_kernel<PREC, 3><<<grid, 256, 0, st>>>(a); break;
+    default: ttm_cols_kernel<PREC, 4><<<grid, 256, 0, st>>>(a); break;
+  }
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+}  // namespace tla
+
+using namespace tla;
+
+extern "C" int tla_khatri_rao(int dtype, const void* const* matrices, int n, const int64_t* rows, int rank,
+                              const void* weights, const void* mask, void* out, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "khatri_rao: bad dtype %d", dtype);
+  TLA_REQUIRE(n >= 1 && n <= kMaxOrder, "khatri_rao: %d matrices (max %d)", n, kMaxOrder);
+  TLA_REQUIRE(rank >= 1, "khatri_rao: rank %d", rank);
+  KrArgs a{};
+  a.n = n; a.R = rank; a.total = 1;
+  for (int m = 0; m < n; ++m) {
+    TLA_REQUIRE(rows[m] >= 0 && matrices[m] != nullptr, "khatri_rao: matrix %d", m);
+    a.rows[m] = rows[m];
+    a.mats[m] = matrices[m];
+    a.total *= rows[m];
+  }
+  a.weights = weights; a.mask = mask;
+  const int64_t ne = a.total * rank;
+  if (ne == 0) return TLA_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TLA_F32) khatri_rao_kernel<float><<<ceil_div(ne, 256), 256, 0, st>>>(a, (float*)out);
+  else khatri_rao_kernel<double><<<ceil_div(ne, 256), 256, 0, st>>>(a, (double*)out);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+extern "C" int tla_mode_dot(int dtype, const void* X, int ndim, const int64_t* shape, const void* U, int64_t J,
+                            int is_vector, int mode, int transpose, int precision, void* out, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "mode_dot: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 1 && ndim <= kMaxOrder, "mode_dot: order %d", ndim);
+  TLA_REQUIRE(mode >= 0 && mode < ndim, "mode_dot: mode %d", mode);
+  TLA_REQUIRE(X && U && out, "mode_dot: NULL pointer");
+  int64_t Pa = 1, Pb = 1;
+  for (int m = 0; m < mode; ++m) Pa *= shape[m];
+  for (int m = mode + 1; m < ndim; ++m) Pb *= shape[m];
+  const int64_t Im = shape[mode];
+  if (is_vector) J = 1;
+  TLA_REQUIRE(J >= 1, "mode_dot: J = %lld", (long long)J);
+  // U'(j, m): matrix J x Im (transpose=0: U row-major J x Im; transpose=1:
+  // U row-major Im x J); vector: U[m].
+  int64_t ldj, ldm;
+  if (is_vector) { ldj = 0; ldm = 1; }
+  else if (transpose) { ldj = 1; ldm = J; }
+  else { ldj = Im; ldm = 1; }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t ne = Pa * J * Pb;
+  if (ne == 0) return TLA_OK;
+  const bool mfma = dtype == TLA_F32 && !is_vector && J <= 64 &&
+                    (precision == TLA_PREC_FP32 || precision == TLA_PREC_BF16 || precision == TLA_PREC_BF16X3);
+  if (mfma && Pb == 1 && Im % 32 == 0) {
+    TtmRowsArgs a{(const float*)X, Pa, Im, (const float*)U, ldj, ldm, (int)J, (float*)out,
+                  (int)std::min<int64_t>(Im, 256)};
+    if (precision == TLA_PREC_FP32) return launch_ttm_rows<TLA_PREC_FP32>(a, st);
+    if (precision == TLA_PREC_BF16) return launch_ttm_rows<TLA_PREC_BF16>(a, st);
+    return launch_ttm_rows<TLA_PREC_BF16X3>(a, st);
+  }
+  if (mfma && Pb > 1 && Pb % 4 == 0 && Pa <= 65535) {
+    TtmColsArgs a{(const float*)X, Pa, Im, Pb, (const float*)U, ldj, ldm, (int)J, (float*)out};
+    if (precision == TLA_PREC_FP32) return launch_ttm_cols<TLA_PREC_FP32>(a, st);
+    if (precision == TLA_PREC_BF16) return launch_ttm_cols<TLA_PREC_BF16>(a, st);
+    return launch_ttm_cols<TLA_PREC_BF16X3>(a, st);
+  }
+  if (dtype == TLA_F32)
+    ttm_generic_kernel<float><<<ceil_div(ne, 256), 256, 0, st>>>((const float*)X, Pa, Im, Pb, (const float*)U, J,
+                                                                 ldj, ldm, (float*)out);
+  else
+    ttm_generic_kernel<double><<<ceil_div(ne, 256), 256, 0, st>>>((const double*)X, Pa, Im, Pb, (const double*)U,
+                                                                  J, ldj, ldm, (double*)out);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+extern "C" size_t tla_rankdiag_contract_workspace_size(int64_t I, int64_t J, int rank, int axis) {
+  if (axis == 1) return 0;
+  const int64_t splits = std::min<int64_t>(I, 64);
+  return align256(size_t(splits) * J * rank * 4);
+}
+
+extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const float* F, int axis,
+                                     float* out, void* ws, size_t ws_bytes, void* stream) {
+  TLA_REQUIRE(rank >= 1 && rank <= 256, "rankdiag: rank %d", rank);
+  TLA_REQUIRE(axis == 0 || axis == 1, "rankdiag: axis %d", axis);
+  TLA_REQUIRE(ws_bytes >= tla_rankdiag_contract_workspace_size(I, J, rank, axis), "rankdiag: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  if (axis == 1) {
+    TLA_REQUIRE(I <= 0x7fffffff, "rankdiag: I too large");
+    rankdiag_axis1_kernel<<<(unsigned)I, 256, 0, st>>>(Y, J, rank, F, out);
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
+  const int splits = (int)std::min<int64_t>(I, 64);
+  const int64_t i_per = (I + splits - 1) / splits;
+  const int64_t n = J * rank;
+  dim3 grid((unsigned)ceil_div(n, 256), (unsigned)splits);
+  rankdiag_axis0_kernel<<<grid, 256, 0, st>>>(Y, I, J, rank, F, i_per, (float*)ws);
+  TLA_LAUNCH_CHECK();
+  sum_splits_kernel<<<ceil_div(n, 256), 256, 0, st>>>((const float*)ws, n, splits, out);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}

@@ -1,0 +1,162 @@
+"""SVD interface on the GPU (``tensorly/tenalg/svd.py``, 0.9.0).
+
+The leading left singular vectors of ``A`` (m x q) are taken as the leading
+eigenvectors of the FP64 Gram ``A A^T`` (or of ``unfold(X, n) unfold(X, n)^T``
+for a tensor unfolding, formed without the copy), computed by the
+``tla_sym_eig`` kernel; singular values are the square roots of the
+eigenvalues.  :func:`svd_flip` applies the reference sign rule
+(``svd.py:13-65``): the largest-magnitude entry of each U column (first one
+on ties) is made positive.
+"""
+import warnings
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, shape_array, stream, workspace
+
+__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "gram_eigh", "mode_gram", "SVD_FUNS"]
+
+SVD_FUNS = ["truncated_svd", "symeig_svd"]
+
+_sigs_added = False
+
+
+def _add_sigs():
+    global _sigs_added
+    if _sigs_added:
+        return
+    L = lib()
+    L.tla_mode_gram64_workspace_size.restype = _lib.c_sz
+    L.tla_mode_gram64_workspace_size.argtypes = [_lib.c_i, _lib.P_i64, _lib.c_i]
+    L.tla_mode_gram64.restype = _lib.c_i
+    L.tla_mode_gram64.argtypes = [_lib.c_i, _lib.c_p, _lib.c_i, _lib.P_i64, _lib.c_i, _lib.c_p, _lib.c_p,
+                                  _lib.c_sz, _lib.c_p]
+    L.tla_sym_eig_workspace_size.restype = _lib.c_sz
+    L.tla_sym_eig_workspace_size.argtypes = [_lib.c_i]
+    L.tla_sym_eig.restype = _lib.c_i
+    L.tla_sym_eig.argtypes = [_lib.c_p, _lib.c_i, _lib.c_i, _lib.c_p, _lib.c_i, _lib.c_p, _lib.c_p, _lib.c_d,
+                              _lib.c_i, _lib.c_p, _lib.c_p, _lib.c_sz, _lib.c_p]
+    _sigs_added = True
+
+
+def mode_gram(tensor, mode):
+    """FP64 ``unfold(X, mode) @ unfold(X, mode)^T`` without forming the
+    unfolding."""
+    _add_sigs()
+    _lib.require_gpu_tensor(tensor, "tensor")
+    x = tensor.contiguous()
+    n = x.shape[mode]
+    sh, _k = shape_array(x.shape)
+    nb = lib().tla_mode_gram64_workspace_size(x.dim(), sh, mode)
+    ws = workspace(nb, x.device)
+    G = torch.empty((n, n), dtype=torch.float64, device=x.device)
+    check(lib().tla_mode_gram64(_lib.dtype_code(x), ptr(x), x.dim(), sh, mode, ptr(G), ptr(ws), nb,
+                                stream(x.device)), "mode_gram64")
+    return G
+
+
+def gram_eigh(G, k, warm=None, tol=1e-11, max_iter=300):
+    """Leading ``k`` eigenpairs (descending) of the symmetric FP64 ``G``:
+    returns ``(evals (k,), evecs (n, k))`` as FP64 GPU tensors."""
+    _add_sigs()
+    n = G.shape[0]
+    G = G.to(torch.float64).contiguous()
+    evals = torch.empty(k, dtype=torch.float64, device=G.device)
+    evecs = torch.empty((n, k), dtype=torch.float64, device=G.device)
+    q0 = warm.to(torch.float64).contiguous() if warm is not None else None
+    nb = lib().tla_sym_eig_workspace_size(n)
+    ws = workspace(nb, G.device)
+    check(lib().tla_sym_eig(ptr(G), n, k, ptr(q0), q0.shape[1] if q0 is not None else 0, ptr(evals), ptr(evecs),
+                            float(tol), int(max_iter), None, ptr(ws), nb, stream(G.device)), "sym_eig")
+    return evals, evecs
+
+
+def svd_flip(U, V, u_based_decision=True):
+    """Sign rule of ``svd.py:13-65`` (on GPU tensors)."""
+    if u_based_decision:
+        idx = torch.argmax(torch.abs(U), dim=0)
+        signs = torch.sign(U[idx, torch.arange(U.shape[1], device=U.device)])
+        U = U * signs
+        if V is not None:
+            if V.shape[0] > U.shape[1]:
+                signs = torch.cat((signs, torch.ones(V.shape[0] - U.shape[1], dtype=V.dtype, device=V.device)))
+            V = V * signs[: V.shape[0]][:, None]
+    else:
+        idx = torch.argmax(torch.abs(V), dim=1)
+        signs = torch.sign(V[torch.arange(V.shape[0], device=V.device), idx])
+        V = V * signs[:, None]
+        if U.shape[1] > V.shape[0]:
+            signs = torch.cat((signs, torch.ones(U.shape[1] - V.shape[0], dtype=V.dtype, device=V.device)))
+        U = U * signs[: U.shape[1]]
+    return U, V
+
+
+def svd_checks(matrix, n_eigenvecs=None):
+    """``svd.py:173-208``."""
+    if matrix.dim() != 2:
+        raise ValueError(f"matrix be a matrix. matrix.ndim is {matrix.dim()} != 2")
+    d1, d2 = matrix.shape
+    lo, hi = min(d1, d2), max(d1, d2)
+    if n_eigenvecs is None:
+        n_eigenvecs = hi
+    if n_eigenvecs > hi:
+        warnings.warn(f"Trying to compute SVD with n_eigenvecs={n_eigenvecs}, which is larger "
+                      f"than max(matrix.shape)={hi}. Setting n_eigenvecs to {hi}.")
+        n_eigenvecs = hi
+    return n_eigenvecs, lo, hi
+
+
+def _left_svd_from_gram(G, n_rows, min_dim, n_eig, warm=None):
+    """U (n_rows x min(n_eig, n_rows)), S (min(n_eig, min_dim)) from the row
+    Gram."""
+    k = min(n_eig, n_rows)
+    evals, evecs = gram_eigh(G, k, warm=warm)
+    s = torch.sqrt(torch.clamp(evals, min=0.0))
+    return evecs, s[: min(n_eig, min_dim)]
+
+
+def truncated_svd(matrix, n_eigenvecs=None, warm=None, **kwargs):
+    """``svd.py:211-235`` semantics: U (m x n_eig), S, V (n_eig x q) with
+    ``V = diag(1/S) U^T A`` (rows with S == 0 left zero)."""
+    n_eig, lo, _ = svd_checks(matrix, n_eigenvecs)
+    A = matrix.contiguous()
+    G = mode_gram(A, 0)
+    U, S = _left_svd_from_gram(G, A.shape[0], lo, n_eig, warm=warm)
+    U = U.to(A.dtype)
+    k = S.shape[0]
+    from .tenalg import mode_dot
+    V = mode_dot(A, U[:, :k], 0, transpose=True)  # k x q
+    inv = torch.where(S > 0, 1.0 / torch.where(S > 0, S, torch.ones_like(S)), torch.zeros_like(S))
+    V = V * inv.to(V.dtype)[:, None]
+    return U, S.to(A.dtype), V
+
+
+def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True, u_based_flip_sign=True,
+                  non_negative=None, mask=None, n_iter_mask_imputation=5, **kwargs):
+    """``svd.py:366-447`` (no mask imputation / non-negativity)."""
+    if mask is not None:
+        raise NotImplementedError("mask imputation is not supported by tensorly_amd")
+    if non_negative:
+        raise NotImplementedError("non-negative SVD is not supported by tensorly_amd")
+    if method not in ("truncated_svd", "symeig_svd") and not callable(method):
+        raise ValueError(f"Got svd={method}. However, the possible choices are {SVD_FUNS} or to pass a callable.")
+    if callable(method):
+        U, S, V = method(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
+    else:
+        U, S, V = truncated_svd(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
+    if flip_sign:
+        U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
+    return U, S, V
+
+
+def leading_left_singular_vectors(tensor, mode, k, warm=None):
+    """Leading ``k`` left singular vectors of ``unfold(tensor, mode)`` with the
+    reference sign rule, and the singular values -- without forming the
+    unfolding (the HOOI / init path)."""
+    n = tensor.shape[mode]
+    q = tensor.numel() // max(n, 1)
+    G = mode_gram(tensor, mode)
+    U, S = _left_svd_from_gram(G, n, min(n, q), k, warm=warm)
+    U, _ = svd_flip(U, None)
+    return U.to(tensor.dtype), S.to(tensor.dtype)

@@ -1,0 +1,89 @@
+"""The C-ABI library loads and exports every symbol include/tensorly_amd.h
+declares; host-side argument checks raise before any GPU work; there is no
+CPU path.  No compute call is made (runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "tensorly_amd.h")) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tla_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from tensorly_amd import _lib
+    lib = _lib.lib()
+    names = _declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.tla_abi_version() == 1
+
+
+def test_workspace_queries_are_host_only():
+    from tensorly_amd import _lib
+    lib = _lib.lib()
+    sh, _k = _lib.shape_array((512, 512, 512))
+    for mode in range(3):
+        for prec in (_lib.PREC_FP32, _lib.PREC_BF16, _lib.PREC_BF16X3):
+            nb = lib.tla_mttkrp_workspace_size(_lib.F32, 3, sh, 32, mode, prec)
+            assert 0 < nb < 64 << 20
+    assert lib.tla_cp_als_workspace_size(_lib.F32, 3, sh, 32, _lib.PREC_BF16X3, 1) > 0
+    assert lib.tla_cp_als_dimtree_supported(_lib.F32, 3, sh, 32, _lib.PREC_FP32) == 1
+    # rank above the kernel limit is refused by the driver sizing
+    assert lib.tla_cp_als_workspace_size(_lib.F32, 3, sh, 65, _lib.PREC_FP32, 0) == 0
+
+
+def test_bad_arguments_return_error_codes():
+    from tensorly_amd import _lib
+    lib = _lib.lib()
+    sh, _k = _lib.shape_array((4, 5, 6))
+    pp, _keep = _lib.ptr_array([None, None, None])
+    rc = lib.tla_mttkrp(_lib.F32, ctypes.c_void_p(0), 3, sh, pp, None, 3, 7, 0, None, None, 0, None)
+    assert rc == _lib.ERR_ARG
+    assert b"mode" in lib.tla_last_error()
+    rc = lib.tla_gram64(_lib.F32, ctypes.c_void_p(0), 10, 100, None, None, 0, None)
+    assert rc == _lib.ERR_ARG
+
+
+def test_no_cpu_path():
+    import tensorly_amd as tla
+    X = torch.zeros((3, 4, 5))
+    f = [torch.zeros((s, 2)) for s in X.shape]
+    with pytest.raises(TypeError):
+        tla.unfolding_dot_khatri_rao(X, (None, f), 0)
+    with pytest.raises(TypeError):
+        tla.parafac(X, 2, init="random")
+    with pytest.raises(TypeError):
+        tla.mode_dot(X, torch.zeros((2, 4)), 1)
+
+
+def test_host_validation_matches_reference_errors():
+    import tensorly_amd as tla
+    with pytest.raises(ValueError):
+        tla.khatri_rao([torch.zeros((3, 4)), torch.zeros((3, 4)), torch.zeros((3, 2))])
+    with pytest.raises(ValueError):
+        tla.mode_dot(torch.zeros((2, 4, 5)), torch.zeros((3, 4)), 0)
+    with pytest.raises(ValueError):
+        tla.mode_dot(torch.zeros((2, 4, 5)), torch.zeros((3,)), 1)
+    with pytest.raises(ValueError):
+        tla.mode_dot(torch.zeros((2, 4, 5)), torch.zeros((2, 4, 5)), 0)
+    with pytest.raises(ValueError):
+        tla.CPTensor((torch.ones(3), [torch.ones((4, 3)), torch.ones((5, 2))]))
+    assert tla.validate_cp_rank((4, 5, 6), "same") == 8
+
+
+def test_unsupported_features_raise():
+    import tensorly_amd as tla
+    X = torch.zeros((3, 4, 5))
+    with pytest.raises(NotImplementedError):
+        tla.parafac(X, 2, mask=torch.ones_like(X))

@@ -1,0 +1,161 @@
+"""parafac / tucker drop-ins on the GPU against the oracle and the
+reference trajectories (tests/golden/trajectories.npz)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import tensorly_np as T
+from tensorly_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def g(x, dev, dtype=torch.float64):
+    return torch.tensor(np.asarray(x), dtype=dtype, device=dev)
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_parafac_fp64_matches_reference_trajectory(cuda, trajectories):
+    """64^3 float64, rank 8, init='random' seed 0, 50 sweeps, tol=0."""
+    import tensorly_amd as tla
+    x, _ = synthetic.cp_64_rank8()
+    cp, errs = tla.parafac(g(x, cuda), 8, n_iter_max=50, init="random", random_state=0, tol=0,
+                           return_errors=True)
+    np.testing.assert_allclose(errs, trajectories["c64_random_errors"], rtol=1e-7)
+    rec = tla.cp_to_tensor(cp).cpu().numpy()
+    ref = T.cp_to_tensor((trajectories["c64_random_weights"],
+                          [trajectories[f"c64_random_factor{m}"] for m in range(3)]))
+    assert rel(rec, ref) < 1e-6
+
+
+@pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("fp32", True), ("bf16x3", False), ("bf16x3", True)])
+def test_parafac_float32_vs_oracle(cuda, prec, dimtree):
+    """float32 tensor (64x48x96 rank-6 model + 1 % noise): per-sweep errors
+    within 1e-5 of the FP64 oracle run on the same data and init."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(3)
+    f_true = [rng.random((s, 6)) for s in (64, 48, 96)]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(6), f_true, dtype=np.float32), 0.01, rng)
+    (_, _), ref_errs = T.parafac(x.astype(np.float64), 6, n_iter_max=25, init="random", random_state=4, tol=0,
+                                 return_errors=True)
+    cp, errs = tla.parafac(g(x, cuda, torch.float32), 6, n_iter_max=25, init="random", random_state=4, tol=0,
+                           return_errors=True, precision=prec, dimtree=dimtree)
+    assert len(errs) == 25
+    assert np.max(np.abs(np.array(errs) - np.array(ref_errs))) < 1e-5, (prec, dimtree)
+
+
+def test_parafac_order4_dimtree(cuda):
+    import tensorly_amd as tla
+    rng = np.random.default_rng(5)
+    shape = (16, 24, 16, 32)
+    f_true = [rng.standard_normal((s, 5)) for s in shape]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(5), f_true, dtype=np.float32), 0.01, rng)
+    (_, _), ref_errs = T.parafac(x.astype(np.float64), 5, n_iter_max=20, init="random", random_state=2, tol=0,
+                                 return_errors=True)
+    for dimtree in (False, True):
+        _, errs = tla.parafac(g(x, cuda, torch.float32), 5, n_iter_max=20, init="random", random_state=2, tol=0,
+                              return_errors=True, dimtree=dimtree)
+        assert np.max(np.abs(np.array(errs) - np.array(ref_errs))) < 1e-5
+
+
+def test_parafac_linesearch_matches_reference(cuda, trajectories):
+    import tensorly_amd as tla
+    x, _ = synthetic.cp_64_rank8()
+    _, errs = tla.parafac(g(x, cuda), 8, n_iter_max=30, init="random", random_state=0, tol=0, linesearch=True,
+                          return_errors=True)
+    np.testing.assert_allclose(errs, trajectories["c64_ls_errors"], rtol=1e-7)
+
+
+def test_parafac_svd_init_normalize_tol(cuda, trajectories):
+    import tensorly_amd as tla
+    x4 = trajectories["x4"]
+    w0, f0 = tla.decomposition.initialize_cp(g(x4, cuda), 5, init="svd")
+    for m in range(4):
+        np.testing.assert_allclose(f0[m].cpu().numpy(), trajectories[f"x4_svdinit_factor{m}"], rtol=1e-6,
+                                   atol=1e-8)
+    cp, errs = tla.parafac(g(x4, cuda), 5, n_iter_max=40, init="svd", tol=1e-9, normalize_factors=True,
+                           return_errors=True)
+    ref = trajectories["x4_svd_norm_errors"]
+    assert len(errs) == len(ref)
+    np.testing.assert_allclose(errs, ref, rtol=1e-6)
+    np.testing.assert_allclose(cp.weights.cpu().numpy(), trajectories["x4_svd_norm_weights"], rtol=1e-5)
+
+
+def test_parafac_fixed_modes_l2_callback(cuda):
+    import tensorly_amd as tla
+    rng = np.random.default_rng(6)
+    x = rng.random((12, 16, 20))
+    seen = []
+
+    def cb(cp, err):
+        seen.append(err)
+        return len(seen) == 4
+
+    ref_cp, ref_errs = T.parafac(x, 3, n_iter_max=10, init="random", random_state=1, tol=0, l2_reg=0.1,
+                                 fixed_modes=[0], return_errors=True)
+    cp, errs = tla.parafac(g(x, cuda), 3, n_iter_max=10, init="random", random_state=1, tol=0, l2_reg=0.1,
+                           fixed_modes=[0], return_errors=True)
+    np.testing.assert_allclose(errs, ref_errs, rtol=1e-9)
+    init0 = np.random.RandomState(1).random_sample((12, 3))
+    np.testing.assert_array_equal(cp.factors[0].cpu().numpy(), init0)
+    _ = tla.parafac(g(x, cuda), 3, n_iter_max=10, init="random", random_state=1, tol=0, callback=cb)
+    assert len(seen) == 4
+
+
+@pytest.mark.parametrize("n,k", [(40, 5), (64, 64), (200, 32), (256, 56)])
+def test_gram_eigensolver(cuda, n, k):
+    from tensorly_amd.svd import gram_eigh
+    rng = np.random.default_rng(n)
+    # spectrum with a gap after k (like a Tucker model plus noise)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    lam = np.concatenate([np.sort(rng.random(k) + 1.0)[::-1] * 10, rng.random(n - k) * 0.5])
+    G = (Q * lam) @ Q.T
+    evals, evecs = gram_eigh(g(G, cuda), k)
+    ref_vals, ref_vecs = np.linalg.eigh(G)
+    ref_vals, ref_vecs = ref_vals[::-1][:k], ref_vecs[:, ::-1][:, :k]
+    np.testing.assert_allclose(evals.cpu().numpy(), ref_vals, rtol=1e-10)
+    overlap = np.abs(np.sum(evecs.cpu().numpy() * ref_vecs, axis=0))
+    np.testing.assert_allclose(overlap, np.ones(k), atol=1e-8)
+
+
+def test_svd_interface_sign_rule(cuda):
+    import tensorly_amd.svd as S
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((30, 50))
+    U, s, V = S.svd_interface(g(A, cuda), n_eigenvecs=6)
+    Ur, sr, Vr = T.svd_interface(A, n_eigenvecs=6)
+    np.testing.assert_allclose(s.cpu().numpy(), sr, rtol=1e-10)
+    np.testing.assert_allclose(U.cpu().numpy(), Ur, atol=1e-9)
+    np.testing.assert_allclose(V.cpu().numpy(), Vr, atol=1e-9)
+
+
+def test_tucker_matches_reference_trajectory(cuda, trajectories):
+    import tensorly_amd as tla
+    xt = trajectories["xt"]
+    tt, errs = tla.tucker(g(xt, cuda), rank=[4, 5, 3], n_iter_max=10, tol=0, return_errors=True)
+    np.testing.assert_allclose(errs, trajectories["xt_errors"], rtol=1e-9)
+    np.testing.assert_allclose(tt.core.cpu().numpy(), trajectories["xt_core"], rtol=1e-7, atol=1e-9)
+    for m in range(3):
+        np.testing.assert_allclose(tt.factors[m].cpu().numpy(), trajectories[f"xt_factor{m}"], atol=1e-8)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_tucker_float32_vs_oracle(cuda, prec, tol):
+    import tensorly_amd as tla
+    x, _ = synthetic.tucker_256_core32(dtype=np.float32, size=96, rank=12)
+    (core, fac), ref_errs = T.tucker(x.astype(np.float64), rank=[12, 12, 12], n_iter_max=5, tol=0,
+                                     return_errors=True)
+    tt, errs = tla.tucker(g(x, cuda, torch.float32), rank=[12, 12, 12], n_iter_max=5, tol=0, return_errors=True,
+                          precision=prec)
+    np.testing.assert_allclose(errs, ref_errs, rtol=tol)
+    for m in range(3):
+        # principal angles between subspaces
+        s = np.linalg.svd(fac[m].T @ tt.factors[m].cpu().numpy().astype(np.float64), compute_uv=False)
+        assert np.min(s) > 1 - 1e-6
+    # the core in the reference's basis (sign rule makes the bases agree)
+    assert rel(tt.core.cpu().numpy(), core) < 10 * tol
