@@ -275,6 +275,502 @@ __global__ __launch_bounds__(256) void mttkrp_inner_kernel(InnerArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// MFMA row-tile kernel.  Rows p of X (all modes but the contiguous one,
+// flattened) are split into units of U = 16 * tpu consecutive rows; wave u
+// computes D = X[rows] @ F_k for its tpu tiles (the F_k panel in LDS is read
+// once per k-step and feeds all tpu tiles) and then either
+//   EPI_REDUCE: out_partial[u, r] = s_q[r] * sum_{rows} F_j[row % J, r] D[row, r]
+//               (q = row / J is constant over a unit because J % U == 0), or
+//   EPI_ROWS:   out[row, r] = w[r] * D[row, r]   (dimension-tree contraction).
+// ---------------------------------------------------------------------------
+constexpr int EPI_REDUCE = 0;
+constexpr int EPI_ROWS = 1;
+constexpr int kTilesPerWave = 4;
+
+struct TilesArgs {
+  const void* X;
+  int64_t K, P, J;
+  int R;
+  int tpu;             // tiles (16 rows) per unit, divides J / 16
+  int64_t n_units;
+  int panel;
+  // s_q = w * prod_{a modes} A_m[q_m] * prod_{b modes} A_m[q_m]; q = (a, i, b)
+  int na, nb;
+  int64_t In, Pb;      // q = (a * In + i) * Pb + b
+  int64_t asz[kMaxOrder], bsz[kMaxOrder];
+  const float* afac[kMaxOrder];
+  const float* bfac[kMaxOrder];
+  const float* weights;
+  const float* Fj;     // J x R (EPI_REDUCE)
+  const float* Fk;     // K x R
+  float* out;          // EPI_REDUCE: [n_units][R]; EPI_ROWS: [P][R]
+};
+
+template <int PREC, typename XT, int RT, int EPI>
+__global__ __launch_bounds__(256) void mttkrp_tiles_kernel(TilesArgs args) {
+  constexpr int RP = RT * 16;
+  constexpr int TPW = kTilesPerWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int R = args.R;
+  const int tpu = args.tpu;
+  const int U = 16 * tpu;
+  const int64_t unit = (int64_t)blockIdx.x * 4 + wave;
+  const bool active = unit < args.n_units;
+  const int64_t row0 = unit * U;
+  const int LDF = inner_pitch_f32(args.panel);
+  const int LDB = inner_pitch_bf16(args.panel);
+  float* Ct = (float*)smem;
+  __bf16* Ch = (__bf16*)smem;
+  __bf16* Cl = Ch + RP * LDB;
+  const XT* X = (const XT*)args.X;
+
+  floatx4 d[TPW][RT];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) d[t][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t kb = 0; kb < args.K; kb += args.panel) {
+    const int klen = (int)min((int64_t)args.panel, args.K - kb);
+    __syncthreads();
+    for (int idx = tid; idx < RP * klen; idx += 256) {
+      const int n = idx % RP, kk = idx / RP;
+      const float v = (n < R) ? args.Fk[(kb + kk) * R + n] : 0.0f;
+      if constexpr (PREC == TLA_PREC_FP32) {
+        Ct[n * LDF + kk] = v;
+      } else {
+        const __bf16 h = (__bf16)v;
+        Ch[n * LDB + kk] = h;
+        if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int kk = 0; kk < klen; kk += 32) {
+        floatx8 av[TPW];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          if (t < tpu) {
+            const XT* p = X + (row0 + 16 * t + c) * args.K + kb + kk + 8 * g;
+            if constexpr (sizeof(XT) == 2) {
+              av[t] = __builtin_convertvector(*(const bf16x8*)p, floatx8);
+            } else {
+              av[t] = load_x8<XT>(p);
+            }
+          }
+        }
+        if constexpr (PREC == TLA_PREC_FP32) {
+#pragma unroll
+          for (int nb = 0; nb < RT; ++nb) {
+            const floatx8 bv = load_x8<float>(Ct + (nb * 16 + c) * LDF + kk + 8 * g);
+#pragma unroll
+            for (int t = 0; t < TPW; ++t)
+              if (t < tpu)
+#pragma unroll
+                for (int v = 0; v < 8; ++v) d[t][nb] = mfma_f32_16x16x4(av[t][v], bv[v], d[t][nb]);
+          }
+        } else {
+          bf16x8 ah[TPW], al[TPW];
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            if (t < tpu) {
+              if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(av[t], ah[t], al[t]);
+              else ah[t] = __builtin_convertvector(av[t], bf16x8);
+            }
+          }
+#pragma unroll
+          for (int nb = 0; nb < RT; ++nb) {
+            const int off = (nb * 16 + c) * LDB + kk + 8 * g;
+            const bf16x8 bh = *(const bf16x8*)(Ch + off);
+            bf16x8 bl;
+            if constexpr (PREC == TLA_PREC_BF16X3) bl = *(const bf16x8*)(Cl + off);
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+              if (t < tpu) {
+                d[t][nb] = mfma_bf16_16x16x32(ah[t], bh, d[t][nb]);
+                if constexpr (PREC == TLA_PREC_BF16X3) {
+                  d[t][nb] = mfma_bf16_16x16x32(ah[t], bl, d[t][nb]);
+                  d[t][nb] = mfma_bf16_16x16x32(al[t], bh, d[t][nb]);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!active) return;
+  // d[t][nb][e] = D[row0 + 16 t + 4 g + e][nb * 16 + c]
+  if constexpr (EPI == EPI_ROWS) {
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int r = nb * 16 + c;
+      if (r >= R) continue;
+      const float w = args.weights ? args.weights[r] : 1.0f;
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+        if (t < tpu)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t row = row0 + 16 * t + 4 * g + e;
+            if (row < args.P) args.out[row * R + r] = w * d[t][nb][e];
+          }
+    }
+  } else {
+    const int64_t q = row0 / args.J;
+    const int64_t j0 = row0 - q * args.J;
+    float s[RT];
+    {
+      int64_t b = q % args.Pb;
+      int64_t a = (q / args.Pb) / args.In;
+#pragma unroll
+      for (int nb = 0; nb < RT; ++nb) {
+        const int r = nb * 16 + c;
+        s[nb] = (r < R) ? (args.weights ? args.weights[r] : 1.0f) : 0.0f;
+      }
+#pragma unroll
+      for (int m = kMaxOrder - 1; m >= 0; --m) {
+        if (m < args.na) {
+          const int64_t idx = a % args.asz[m];
+          a /= args.asz[m];
+#pragma unroll
+          for (int nb = 0; nb < RT; ++nb) {
+            const int r = nb * 16 + c;
+            if (r < R) s[nb] *= args.afac[m][idx * R + r];
+          }
+        }
+      }
+#pragma unroll
+      for (int m = kMaxOrder - 1; m >= 0; --m) {
+        if (m < args.nb) {
+          const int64_t idx = b % args.bsz[m];
+          b /= args.bsz[m];
+#pragma unroll
+          for (int nb = 0; nb < RT; ++nb) {
+            const int r = nb * 16 + c;
+            if (r < R) s[nb] *= args.bfac[m][idx * R + r];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int r = nb * 16 + c;
+      float acc = 0.0f;
+      if (r < R) {
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+          if (t < tpu)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc = fmaf(args.Fj[(j0 + 16 * t + 4 * g + e) * R + r], d[t][nb][e], acc);
+      }
+      acc += __shfl_xor(acc, 16, 64);
+      acc += __shfl_xor(acc, 32, 64);
+      if (g == 0 && r < R) args.out[unit * R + r] = s[nb] * acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent interleaved row-tile kernel ("stream").  The whole F_k (K x R)
+// is staged once per block in LDS; wave w of W visits 16-row tiles
+// w, w + W, w + 2W, ... so that at any time the grid reads one narrow window
+// of the tensor (measured: ~6.3 TB/s for this pattern versus ~5.4 TB/s when
+// each wave owns a contiguous slab).  Loads are double-buffered in
+// registers, 64 columns (two MFMA k-steps) per buffer.  Epilogues:
+//   SE_ROWS   out[row, r] = w[r] * D[row, r]
+//   SE_REDQ   partial[tile, r] = s_q[r] * sum_rows F_j[row % J, r] D[row, r]
+//   SE_ACCJ   (output mode = mode N-2, rows of a tile are output rows):
+//             acc[rows] += s_q * D, W % (J/16) == 0 keeps the wave on one
+//             16-row output block; partial[wave, 16, r].
+// ---------------------------------------------------------------------------
+constexpr int SE_ROWS = 0;
+constexpr int SE_REDQ = 1;
+constexpr int SE_ACCJ = 2;
+
+struct StreamArgs {
+  const void* X;
+  int64_t K, P, J;
+  int R;
+  int64_t ntiles;
+  // s_q = w * prod_{a modes} A_m[q_m] * prod_{b modes} A_m[q_m]; q = (a, i, b)
+  int na, nb;
+  int64_t In, Pb;
+  int64_t asz[kMaxOrder], bsz[kMaxOrder];
+  const float* afac[kMaxOrder];
+  const float* bfac[kMaxOrder];
+  const float* weights;
+  const float* Fj;
+  const float* Fk;
+  const float* Sq;  // (ntiles / (J / 16)) x R block scales, precomputed
+  float* out;
+};
+
+template <int RT>
+__device__ __forceinline__ void scale_q(const StreamArgs& a, int64_t q, int c, float s[RT]) {
+  const int R = a.R;
+  int64_t b = q % a.Pb;
+  int64_t aa = (q / a.Pb) / a.In;
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) {
+    const int r = nb * 16 + c;
+    s[nb] = (r < R) ? (a.weights ? a.weights[r] : 1.0f) : 0.0f;
+  }
+#pragma unroll
+  for (int m = kMaxOrder - 1; m >= 0; --m) {
+    if (m < a.na) {
+      const int64_t idx = aa % a.asz[m];
+      aa /= a.asz[m];
+#pragma unroll
+      for (int nb = 0; nb < RT; ++nb) {
+        const int r = nb * 16 + c;
+        if (r < R) s[nb] *= a.afac[m][idx * R + r];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = kMaxOrder - 1; m >= 0; --m) {
+    if (m < a.nb) {
+      const int64_t idx = b % a.bsz[m];
+      b /= a.bsz[m];
+#pragma unroll
+      for (int nb = 0; nb < RT; ++nb) {
+        const int r = nb * 16 + c;
+        if (r < R) s[nb] *= a.bfac[m][idx * R + r];
+      }
+    }
+  }
+}
+
+template <typename XT>
+__device__ __forceinline__ void load_pair(const XT* p, floatx8& v0, floatx8& v1) {
+  if constexpr (sizeof(XT) == 2) {
+    v0 = __builtin_convertvector(*(const bf16x8*)p, floatx8);
+    v1 = __builtin_convertvector(*(const bf16x8*)(p + 32), floatx8);
+  } else {
+    v0 = load_x8<XT>(p);
+    v1 = load_x8<XT>(p + 32);
+  }
+}
+
+template <int PREC, int RT>
+__device__ __forceinline__ void stream_step(const floatx8& av, const float* Ct, const __bf16* Ch, const __bf16* Cl,
+                                            int LDF, int LDB, int kk, int c, int g, floatx4 d[RT]) {
+  if constexpr (PREC == TLA_PREC_FP32) {
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const floatx8 bv = load_x8<float>(Ct + (nb * 16 + c) * LDF + kk + 8 * g);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) d[nb] = mfma_f32_16x16x4(av[v], bv[v], d[nb]);
+    }
+  } else {
+    bf16x8 ah, al;
+    if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(av, ah, al);
+    else ah = __builtin_convertvector(av, bf16x8);
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int off = (nb * 16 + c) * LDB + kk + 8 * g;
+      const bf16x8 bh = *(const bf16x8*)(Ch + off);
+      d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
+      if constexpr (PREC == TLA_PREC_BF16X3) {
+        const bf16x8 bl = *(const bf16x8*)(Cl + off);
+        d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+        d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
+      }
+    }
+  }
+}
+
+constexpr int kStreamWaves = 8;  // 512-thread blocks share one LDS copy of F_k
+
+// One ring slot: a 64-column chunk of a 16-row tile plus, when the chunk is
+// the tile's last, the tile's epilogue operands (loaded in the same batch so
+// the in-order return of vector loads never makes the epilogue wait on the
+// prefetched chunks behind it).
+template <int RT>
+struct StreamSlot {
+  floatx8 a, b;
+  float fj[4][RT];
+  float sq[RT];
+};
+
+template <int PREC, typename XT, int RT, int EPI>
+__device__ __forceinline__ void stream_finish_tile(const StreamArgs& args, int64_t tile, int c, int g,
+                                                   const StreamSlot<RT>& sl, const float wr[RT], floatx4 d[RT],
+                                                   floatx4 acc[RT]) {
+  const int R = args.R;
+  const int64_t row0 = tile * 16;
+  // d[nb][e] = D[row0 + 4 g + e][nb * 16 + c]
+  if constexpr (EPI == SE_ROWS) {
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int r = nb * 16 + c;
+      if (r < R)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) args.out[(row0 + 4 * g + e) * R + r] = wr[nb] * d[nb][e];
+    }
+  } else if constexpr (EPI == SE_REDQ) {
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int r = nb * 16 + c;
+      float t = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) t = fmaf(sl.fj[e][nb], d[nb][e], t);
+      t += __shfl_xor(t, 16, 64);
+      t += __shfl_xor(t, 32, 64);
+      if (g == 0 && r < R) args.out[tile * R + r] = sl.sq[nb] * t;
+    }
+  } else {
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) acc[nb] += sl.sq[nb] * d[nb];
+  }
+}
+
+template <int PREC, typename XT, int RT, int EPI>
+__global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(StreamArgs args) {
+  constexpr int RP = RT * 16;
+  constexpr int NT = 64 * kStreamWaves;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int R = args.R;
+  const int K = (int)args.K;  // K % 64 == 0, K * RP fits LDS
+  const int LDF = K + 4, LDB = K + 8;
+  float* Ct = (float*)smem;
+  __bf16* Ch = (__bf16*)smem;
+  __bf16* Cl = Ch + RP * LDB;
+  for (int idx = tid; idx < RP * K; idx += NT) {
+    const int n = idx % RP, kk = idx / RP;
+    const float v = (n < R) ? args.Fk[(int64_t)kk * R + n] : 0.0f;
+    if constexpr (PREC == TLA_PREC_FP32) {
+      Ct[n * LDF + kk] = v;
+    } else {
+      const __bf16 h = (__bf16)v;
+      Ch[n * LDB + kk] = h;
+      if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+    }
+  }
+  float wr[RT];
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) {
+    const int r = nb * 16 + c;
+    wr[nb] = (r < R && args.weights) ? args.weights[r] : 1.0f;
+  }
+  __syncthreads();
+
+  const int64_t W = (int64_t)gridDim.x * kStreamWaves;
+  const int64_t w = (int64_t)blockIdx.x * kStreamWaves + wave;
+  const XT* X = (const XT*)args.X;
+  const int64_t tpq = args.J / 16;
+  floatx4 acc[RT];
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int64_t my_tiles = (w < args.ntiles) ? (args.ntiles - w + W - 1) / W : 0;
+  const int cpt = K / 64;  // 64-column chunks per tile
+  const int64_t nch = my_tiles * cpt;
+  // chunk i covers tile w + (i / cpt) * W, columns (i % cpt) * 64 .. +63.
+  // Three-deep ring: chunk i + 2 is requested while chunk i is multiplied.
+  int64_t ld_tile = w;
+  int ld_kk = 0;
+  auto issue = [&](StreamSlot<RT>& sl) {
+    load_pair<XT>(X + (ld_tile * 16 + c) * args.K + ld_kk + 8 * g, sl.a, sl.b);
+    if (ld_kk + 64 == K) {
+      if constexpr (EPI != SE_ROWS) {
+        const int64_t q = ld_tile / tpq;
+#pragma unroll
+        for (int nb = 0; nb < RT; ++nb) {
+          const int r = nb * 16 + c;
+          sl.sq[nb] = (r < R) ? args.Sq[q * R + r] : 0.0f;
+        }
+        if constexpr (EPI == SE_REDQ) {
+          const int64_t j0 = ld_tile * 16 - q * args.J;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int nb = 0; nb < RT; ++nb) {
+              const int r = nb * 16 + c;
+              sl.fj[e][nb] = (r < R) ? args.Fj[(j0 + 4 * g + e) * R + r] : 0.0f;
+            }
+        }
+      }
+    }
+    ld_kk += 64;
+    if (ld_kk == K) { ld_kk = 0; ld_tile += W; }
+  };
+  StreamSlot<RT> s0, s1, s2;
+  if (nch > 0) issue(s0);
+  if (nch > 1) issue(s1);
+  int64_t cur_tile = w;
+  int cur_kk = 0;
+  floatx4 d[RT];
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) d[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto consume = [&](const StreamSlot<RT>& sl) {
+    stream_step<PREC, RT>(sl.a, Ct, Ch, Cl, LDF, LDB, cur_kk, c, g, d);
+    stream_step<PREC, RT>(sl.b, Ct, Ch, Cl, LDF, LDB, cur_kk + 32, c, g, d);
+    cur_kk += 64;
+    if (cur_kk == K) {
+      stream_finish_tile<PREC, XT, RT, EPI>(args, cur_tile, c, g, sl, wr, d, acc);
+#pragma unroll
+      for (int nb = 0; nb < RT; ++nb) d[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+      cur_kk = 0;
+      cur_tile += W;
+    }
+  };
+  for (int64_t i = 0; i < nch; i += 3) {
+    if (i + 2 < nch) issue(s2);
+    consume(s0);
+    if (i + 1 >= nch) break;
+    if (i + 3 < nch) issue(s0);
+    consume(s1);
+    if (i + 2 >= nch) break;
+    if (i + 4 < nch) issue(s1);
+    consume(s2);
+  }
+  if constexpr (EPI == SE_ACCJ) {
+    // this wave's output rows: 16 * (w % tpq) + 4 g + e
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const int r = nb * 16 + c;
+      if (r < R)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) args.out[(w * 16 + 4 * g + e) * R + r] = acc[nb][e];
+    }
+  }
+}
+
+// out[j, r] = sum over waves w with w % tpq == j / 16 of partial[w, j % 16, r]
+__global__ void accj_reduce_kernel(const float* __restrict__ partial, int64_t W, int64_t J, int R,
+                                   float* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= J * R) return;
+  const int r = (int)(e % R);
+  const int64_t j = e / R;
+  const int64_t tpq = J / 16;
+  float acc = 0.0f;
+  for (int64_t w = j / 16; w < W; w += tpq) acc += partial[(w * 16 + (j % 16)) * R + r];
+  out[e] = acc;
+}
+
+// M[i, r] = sum_{a, b} sum_{s < S} partial[((a * In + i) * Pb + b) * S + s][r]
+__global__ void tiles_reduce_kernel(const float* __restrict__ partial, int64_t Pa, int64_t In, int64_t Pb, int S,
+                                    int R, float* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= In * R) return;
+  const int r = (int)(e % R);
+  const int64_t i = e / R;
+  float acc = 0.0f;
+  for (int64_t a = 0; a < Pa; ++a)
+    for (int64_t b = 0; b < Pb; ++b) {
+      const int64_t u0 = ((a * In + i) * Pb + b) * S;
+      for (int s = 0; s < S; ++s) acc += partial[(u0 + s) * R + r];
+    }
+  out[e] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // MFMA last-mode kernel
 // ---------------------------------------------------------------------------
 struct LastArgs {
@@ -413,6 +909,154 @@ __global__ __launch_bounds__(512) void mttkrp_last_kernel(LastArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent interleaved last-mode kernel.  A block of 8 waves covers up to
+// 8 column tiles of 64 output columns (one column group) and
+// rsplit = 8 / (waves per row) row sub-ranges of 32 rows; blocks of a column
+// group take row visits of 32 * rsplit rows interleaved across the grid, so
+// the grid sweeps the tensor front to back.  Each wave writes its own
+// partial [slot][Kc][R], slot = block-in-group * rsplit + sub-range.
+// ---------------------------------------------------------------------------
+struct LastStreamArgs {
+  const void* X;
+  int64_t K, P, J;
+  int R;
+  const float* Fj;
+  const float* Gm;
+  float* partial;
+  int wpr, rsplit;
+  int64_t G;        // blocks per column group
+  int64_t nvisits;  // ceil(P / (32 * rsplit))
+};
+
+// One visit's operands: 8 rows of this lane's column quad plus the Khatri-Rao
+// row factors of those rows (requested together, see StreamSlot).
+template <int RT>
+struct LastSlot {
+  floatx4 x[8];
+  float gm[RT];
+  float fj[8][RT];
+};
+
+template <typename XT, int RT>
+__device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* X, int64_t col, bool colok,
+                                          int64_t rowb, int64_t pe, int c, LastSlot<RT>& sl) {
+  const int R = args.R;
+  const int64_t K = args.K, J = args.J;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int64_t row = rowb + u;
+    sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (row < pe && colok) sl.x[u] = load_x4<XT>(X + row * K + col);
+  }
+  // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0
+  const bool okrow = rowb < pe;
+  const int64_t aa = okrow ? rowb / J : 0;
+  const int64_t jj = rowb - aa * J;
+#pragma unroll
+  for (int mb = 0; mb < RT; ++mb) {
+    const int r = mb * 16 + c;
+    sl.gm[mb] = (okrow && r < R) ? args.Gm[aa * R + r] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl.fj[u][mb] = ((rowb + u) < pe && r < R) ? args.Fj[(jj + u) * R + r] : 0.0f;
+  }
+}
+
+template <int PREC, int RT>
+__device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4][RT]) {
+  floatx8 w8[RT];
+#pragma unroll
+  for (int mb = 0; mb < RT; ++mb)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) w8[mb][u] = sl.gm[mb] * sl.fj[u][mb];
+  if constexpr (PREC == TLA_PREC_FP32) {
+    // MFMA u contracts rows rowb(g) + u across the four lane groups
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mb = 0; mb < RT; ++mb) acc[t][mb] = mfma_f32_16x16x4(w8[mb][u], sl.x[u][t], acc[t][mb]);
+  } else {
+    bf16x8 wh[RT], wl[RT];
+#pragma unroll
+    for (int mb = 0; mb < RT; ++mb) {
+      if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(w8[mb], wh[mb], wl[mb]);
+      else wh[mb] = __builtin_convertvector(w8[mb], bf16x8);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      floatx8 x8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x8[u] = sl.x[u][t];
+      bf16x8 xh, xl;
+      if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(x8, xh, xl);
+      else xh = __builtin_convertvector(x8, bf16x8);
+#pragma unroll
+      for (int mb = 0; mb < RT; ++mb) {
+        acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xh, acc[t][mb]);
+        if constexpr (PREC == TLA_PREC_BF16X3) {
+          acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xl, acc[t][mb]);
+          acc[t][mb] = mfma_bf16_16x16x32(wl[mb], xh, acc[t][mb]);
+        }
+      }
+    }
+  }
+}
+
+template <int PREC, typename XT, int RT>
+__global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs args) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int R = args.R;
+  const int64_t K = args.K;
+  const int64_t cgroup = blockIdx.x / args.G;
+  const int64_t b = blockIdx.x - cgroup * args.G;
+  const int wc = wave % args.wpr, rs = wave / args.wpr;
+  if (rs >= args.rsplit) return;  // only when wpr * rsplit < 8; no block-wide sync follows
+  const int64_t ctile = cgroup * 8 + wc;
+  const int64_t col = ctile * 64 + 4 * c;
+  const bool colok = col < K;
+  const XT* X = (const XT*)args.X;
+  floatx4 acc[4][RT];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int mb = 0; mb < RT; ++mb) acc[t][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int64_t V = 32 * (int64_t)args.rsplit;
+  // visit v covers rows v * V + 32 * rs .. +31; lane rows p0 + 8 g + u.
+  auto rowb_of = [&](int64_t v) { return v * V + 32 * rs + 8 * g; };
+  auto pe_of = [&](int64_t v) { return min(args.P, v * V + 32 * rs + 32); };
+  LastSlot<RT> sa, sb;
+  int64_t va = b;
+  if (va < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(va), pe_of(va), c, sa);
+  for (; va < args.nvisits; va += 2 * args.G) {
+    const int64_t vb = va + args.G;
+    if (vb < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vb), pe_of(vb), c, sb);
+    last_visit<PREC, RT>(sa, acc);
+    if (vb >= args.nvisits) break;
+    const int64_t vc = vb + args.G;
+    if (vc < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vc), pe_of(vc), c, sa);
+    last_visit<PREC, RT>(sb, acc);
+  }
+  // acc[t][mb][e] = out[col + t][mb * 16 + 4 g + e]
+  const int64_t slot = b * args.rsplit + rs;
+  float* dst = args.partial + slot * K * R;
+  if (colok) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t k = col + t;
+#pragma unroll
+      for (int mb = 0; mb < RT; ++mb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = mb * 16 + 4 * g + e;
+          if (r < R) dst[k * R + r] = acc[t][mb][e];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Split reduction and helpers
 // ---------------------------------------------------------------------------
 __global__ void mttkrp_reduce_kernel(const float* __restrict__ partial, int64_t n, int splits,
@@ -446,7 +1090,7 @@ __global__ void build_gm_kernel(GmArgs args, float* __restrict__ gm) {
     if (m < args.n) {
       const int64_t idx = a % args.sz[m];
       a /= args.sz[m];
-      v *= args.fac[m][idx * args.R + r];
+      if (args.fac[m]) v *= args.fac[m][idx * args.R + r];
     }
   }
   gm[e] = v;
@@ -455,11 +1099,21 @@ __global__ void build_gm_kernel(GmArgs args, float* __restrict__ gm) {
 // ---------------------------------------------------------------------------
 // Host planning
 // ---------------------------------------------------------------------------
-enum class Path { Generic, Inner, Last };
+enum class Path { Generic, Inner, Tiles, Last, Stream, LastStream };
 
 struct Plan {
   Path path;
   int RT;
+  // stream
+  StreamArgs st;
+  int sepi;
+  int64_t s_W;
+  LastStreamArgs ls;
+  // tiles
+  TilesArgs tl;
+  int epi;
+  int64_t t_Pa, t_In, t_Pb;
+  int t_S;
   // inner
   InnerArgs in;
   dim3 grid;
@@ -515,6 +1169,111 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
       p->ws_bytes = 0;
       return TLA_OK;
     }
+    // Row-tile kernel when mode N-2 is contracted (rows reduce into the
+    // output) or when every non-contiguous mode is kept (rows are the output).
+    const int64_t J = shape[ndim - 2];
+    int64_t Prow = 1;
+    for (int m = 0; m < ndim - 1; ++m) Prow *= shape[m];
+    {
+      // Persistent interleaved kernel when F_k fits LDS whole.
+      const size_t lds = (precision == TLA_PREC_FP32) ? size_t(RT * 16) * (K + 4) * 4
+                                                       : size_t(2) * RT * 16 * (K + 8) * 2;
+      const bool s_rows = (kb == 0 && ke == ndim - 1);
+      const bool s_redq = (ke <= ndim - 2 && J % 16 == 0);
+      const bool s_accj = (standard && kb == ndim - 2 && J % 16 == 0);
+      const bool off = getenv("TLA_NO_STREAM") != nullptr;
+      if (!off && K % 64 == 0 && Prow % 16 == 0 && lds <= 80 * 1024 && (s_rows || s_redq || s_accj)) {
+        p->path = Path::Stream;
+        p->sepi = s_rows ? SE_ROWS : (s_redq ? SE_REDQ : SE_ACCJ);
+        StreamArgs& s = p->st;
+        s.K = K; s.P = Prow; s.J = J; s.R = R;
+        s.ntiles = Prow / 16;
+        const int64_t tpq = J / 16;
+        constexpr int NW = kStreamWaves;
+        int64_t blocks = 256;
+        if (const char* e = getenv("TLA_STREAM_BLOCKS")) blocks = std::max(1, atoi(e));
+        blocks = std::min<int64_t>(blocks, (s.ntiles + NW - 1) / NW);
+        int64_t Wv = blocks * NW;
+        if (p->sepi == SE_ACCJ) {
+          int64_t L = tpq;
+          while (L % NW) L += tpq;  // lcm(waves per block, tpq)
+          Wv = std::max<int64_t>(L, (Wv + L / 2) / L * L);
+          blocks = Wv / NW;
+        }
+        p->s_W = Wv;
+        if (p->sepi == SE_ACCJ) {
+          s.na = ndim - 2; s.nb = 0; s.In = 1; s.Pb = 1;
+          for (int m = 0; m < ndim - 2; ++m) s.asz[m] = shape[m];
+        } else {
+          int64_t Pbr = 1;
+          for (int m = ke; m < ndim - 2; ++m) Pbr *= shape[m];
+          s.na = kb;
+          s.nb = s_rows ? 0 : (ndim - 2 - ke);
+          for (int m = 0; m < kb; ++m) s.asz[m] = shape[m];
+          for (int m = ke; m < ndim - 2; ++m) s.bsz[m - ke] = shape[m];
+          s.In = In;
+          s.Pb = s_rows ? 1 : Pbr;
+          p->t_Pa = Pa; p->t_In = In; p->t_Pb = Pbr; p->t_S = (int)tpq;
+        }
+        p->grid = dim3((unsigned)blocks, 1, 1);
+        p->smem = lds;
+        p->partial_elems = (p->sepi == SE_ROWS) ? 0
+                           : (p->sepi == SE_REDQ) ? size_t(s.ntiles) * R : size_t(Wv) * 16 * R;
+        // block scales Sq[q, r] = w[r] * prod of the factor rows of q's digits
+        // (the output digit excluded), one row per 16 * tpq-row block
+        GmArgs& sq = p->gm;
+        sq = GmArgs{};
+        sq.R = R;
+        if (p->sepi == SE_ACCJ) {
+          sq.n = ndim - 2;
+          for (int m = 0; m < ndim - 2; ++m) sq.sz[m] = shape[m];
+        } else if (p->sepi == SE_REDQ) {
+          int n = 0;
+          for (int m = 0; m < kb; ++m) sq.sz[n++] = shape[m];
+          sq.sz[n++] = In;
+          for (int m = ke; m < ndim - 2; ++m) sq.sz[n++] = shape[m];
+          sq.n = n;
+        }
+        p->gm_rows = (p->sepi == SE_ROWS) ? 0 : s.ntiles / tpq;
+        sq.rows = p->gm_rows;
+        p->ws_bytes = align256(size_t(p->gm_rows) * R * 4) + align256(p->partial_elems * 4);
+        return TLA_OK;
+      }
+    }
+    const bool rows_epi = (kb == 0 && ke == ndim - 1 && Prow % 16 == 0);
+    const bool reduce_epi = (ke <= ndim - 2 && J % 16 == 0);
+    if (rows_epi || reduce_epi) {
+      p->path = Path::Tiles;
+      p->epi = rows_epi ? EPI_ROWS : EPI_REDUCE;
+      TilesArgs& t = p->tl;
+      t.K = K; t.P = Prow; t.R = R;
+      t.J = rows_epi ? Prow : J;
+      int tpu = 4;
+      if (const char* e = getenv("TLA_TILES_TPU")) tpu = std::max(1, std::min(4, atoi(e)));
+      while (tpu > 1 && (t.J % (16 * tpu)) != 0) tpu >>= 1;
+      t.tpu = tpu;
+      t.n_units = Prow / (16 * tpu);
+      int panel = 128;
+      if (const char* e = getenv("TLA_TILES_PANEL")) panel = std::max(32, atoi(e) / 32 * 32);
+      t.panel = (int)std::min<int64_t>(K, panel);
+      // q = (a, i, b) with a = modes < kb, i = kept block, b = modes ke..N-3
+      int64_t Pbr = 1;
+      for (int m = ke; m < ndim - 2; ++m) Pbr *= shape[m];
+      t.na = kb;
+      t.nb = rows_epi ? 0 : (ndim - 2 - ke);
+      for (int m = 0; m < kb; ++m) t.asz[m] = shape[m];
+      for (int m = ke; m < ndim - 2; ++m) t.bsz[m - ke] = shape[m];
+      t.In = In;
+      t.Pb = rows_epi ? 1 : Pbr;
+      p->t_Pa = Pa; p->t_In = In; p->t_Pb = Pbr;
+      p->t_S = rows_epi ? 1 : (int)(J / (16 * tpu));
+      p->grid = dim3((unsigned)ceil_div(t.n_units, 4), 1, 1);
+      p->smem = (precision == TLA_PREC_FP32) ? size_t(RT * 16) * inner_pitch_f32(t.panel) * 4
+                                             : size_t(2) * RT * 16 * inner_pitch_bf16(t.panel) * 2;
+      p->partial_elems = rows_epi ? 0 : size_t(t.n_units) * R;
+      p->ws_bytes = align256(p->partial_elems * 4);
+      return TLA_OK;
+    }
     p->path = Path::Inner;
     InnerArgs& a = p->in;
     a.K = K; a.In = In; a.Pa = Pa; a.Pb = Pb;
@@ -558,9 +1317,6 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     p->ws_bytes = 0;
     return TLA_OK;
   }
-  p->path = Path::Last;
-  LastArgs& l = p->last;
-  l.K = Kc; l.P = P; l.J = J; l.R = R;
   GmArgs& gm = p->gm;
   gm.n = kb - 1;
   gm.R = R;
@@ -568,9 +1324,36 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
   for (int m = 0; m < kb - 1; ++m) { gm.sz[m] = shape[m]; gm_rows *= shape[m]; }
   gm.rows = gm_rows;
   p->gm_rows = gm_rows;
+  if (getenv("TLA_NO_STREAM") == nullptr && J % 8 == 0) {
+    p->path = Path::LastStream;
+    LastStreamArgs& s = p->ls;
+    s.K = Kc; s.P = P; s.J = J; s.R = R;
+    const int64_t ktiles = ceil_div(Kc, 64);
+    int wpr = 1;
+    while (wpr < 8 && wpr < ktiles) wpr <<= 1;
+    s.wpr = wpr;
+    s.rsplit = 8 / wpr;
+    const int64_t cgroups = (wpr == 8) ? ceil_div(ktiles, 8) : 1;
+    s.nvisits = (P + 32 * s.rsplit - 1) / (32 * s.rsplit);
+    int64_t target = 256;
+    if (const char* e = getenv("TLA_LAST_STREAM_BLOCKS")) target = std::max(1, atoi(e));
+    int64_t G = std::max<int64_t>(1, target / cgroups);
+    G = std::min<int64_t>(G, s.nvisits);
+    s.G = G;
+    p->grid = dim3((unsigned)(G * cgroups), 1, 1);
+    p->splits = (int)(G * s.rsplit);
+    p->partial_elems = size_t(p->splits) * Kc * R;
+    p->ws_bytes = align256(size_t(gm_rows) * R * 4) + align256(p->partial_elems * 4);
+    return TLA_OK;
+  }
+  p->path = Path::Last;
+  LastArgs& l = p->last;
+  l.K = Kc; l.P = P; l.J = J; l.R = R;
   const int ktiles = ceil_div(Kc, 64);
   const int step = precision == TLA_PREC_FP32 ? 4 : 32;
-  int64_t S = std::max<int64_t>(1, (1024 + ktiles - 1) / ktiles);
+  int64_t target_blocks = 512;
+  if (const char* e = getenv("TLA_LAST_BLOCKS")) target_blocks = std::max(1, atoi(e));
+  int64_t S = std::max<int64_t>(1, (target_blocks + ktiles - 1) / ktiles);
   const int64_t max_s = std::max<int64_t>(1, P / (kLastWaves * step));
   S = std::min<int64_t>(S, max_s);
   S = std::min<int64_t>(S, 65535);
@@ -604,6 +1387,83 @@ static int launch_inner(const Plan& p, hipStream_t st) {
   }
   TLA_LAUNCH_CHECK();
   return TLA_OK;
+}
+
+template <int PREC, typename XT, int EPI>
+static int launch_tiles_epi(const Plan& p, hipStream_t st) {
+  TilesArgs a = p.tl;
+  switch (p.RT) {
+    case 1: mttkrp_tiles_kernel<PREC, XT, 1, EPI><<<p.grid, 256, p.smem, st>>>(a); break;
+    case 2: mttkrp_tiles_kernel<PREC, XT, 2, EPI><<<p.grid, 256, p.smem, st>>>(a); break;
+    case 3: mttkrp_tiles_kernel<PREC, XT, 3, EPI><<<p.grid, 256, p.smem, st>>>(a); break;
+    case 4: mttkrp_tiles_kernel<PREC, XT, 4, EPI><<<p.grid, 256, p.smem, st>>>(a); break;
+    default:
+      set_error("rank tile %d unsupported", p.RT);
+      return TLA_ERR_UNSUPPORTED;
+  }
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+// Kernels staging more than 64 KB of dynamic LDS must opt in (gfx950 has
+// 160 KB per workgroup).
+template <typename F>
+static int allow_lds(F* fn, size_t bytes) {
+  if (bytes > 64 * 1024)
+    TLA_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return TLA_OK;
+}
+
+template <int PREC, typename XT, int RT, int EPI>
+static int launch_stream_rt(const Plan& p, hipStream_t st) {
+  auto* fn = &mttkrp_stream_kernel<PREC, XT, RT, EPI>;
+  int rc = allow_lds(fn, p.smem);
+  if (rc) return rc;
+  fn<<<p.grid, 64 * kStreamWaves, p.smem, st>>>(p.st);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+template <int PREC, typename XT, int EPI>
+static int launch_stream_epi(const Plan& p, hipStream_t st) {
+  switch (p.RT) {
+    case 1: return launch_stream_rt<PREC, XT, 1, EPI>(p, st);
+    case 2: return launch_stream_rt<PREC, XT, 2, EPI>(p, st);
+    case 3: return launch_stream_rt<PREC, XT, 3, EPI>(p, st);
+    case 4: return launch_stream_rt<PREC, XT, 4, EPI>(p, st);
+    default:
+      set_error("rank tile %d unsupported", p.RT);
+      return TLA_ERR_UNSUPPORTED;
+  }
+}
+
+template <int PREC, typename XT>
+static int launch_stream(const Plan& p, hipStream_t st) {
+  if (p.sepi == SE_ROWS) return launch_stream_epi<PREC, XT, SE_ROWS>(p, st);
+  if (p.sepi == SE_REDQ) return launch_stream_epi<PREC, XT, SE_REDQ>(p, st);
+  return launch_stream_epi<PREC, XT, SE_ACCJ>(p, st);
+}
+
+template <int PREC, typename XT>
+static int launch_last_stream(const Plan& p, hipStream_t st) {
+  LastStreamArgs a = p.ls;
+  switch (p.RT) {
+    case 1: mttkrp_last_stream_kernel<PREC, XT, 1><<<p.grid, 512, 0, st>>>(a); break;
+    case 2: mttkrp_last_stream_kernel<PREC, XT, 2><<<p.grid, 512, 0, st>>>(a); break;
+    case 3: mttkrp_last_stream_kernel<PREC, XT, 3><<<p.grid, 512, 0, st>>>(a); break;
+    case 4: mttkrp_last_stream_kernel<PREC, XT, 4><<<p.grid, 512, 0, st>>>(a); break;
+    default:
+      set_error("rank tile %d unsupported", p.RT);
+      return TLA_ERR_UNSUPPORTED;
+  }
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+template <int PREC, typename XT>
+static int launch_tiles(const Plan& p, hipStream_t st) {
+  return p.epi == EPI_ROWS ? launch_tiles_epi<PREC, XT, EPI_ROWS>(p, st)
+                           : launch_tiles_epi<PREC, XT, EPI_REDUCE>(p, st);
 }
 
 template <int PREC, typename XT>
@@ -665,6 +1525,90 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     TLA_REQUIRE((m >= kb && m < ke) || factors[m] != nullptr, "factor %d is NULL", m);
   int rc = TLA_OK;
   float* partial;
+  if (p.path == Path::Stream) {
+    StreamArgs& s = p.st;
+    s.X = X;
+    s.weights = (const float*)weights;
+    s.Fj = (const float*)factors[ndim - 2];
+    s.Fk = (const float*)factors[ndim - 1];
+    float* sqbuf = (float*)ws;
+    float* part = (float*)((char*)ws + align256(size_t(p.gm_rows) * R * 4));
+    if (p.sepi != SE_ROWS) {
+      GmArgs g = p.gm;
+      if (p.sepi == SE_ACCJ) {
+        for (int m = 0; m < g.n; ++m) g.fac[m] = (const float*)factors[m];
+      } else {
+        int n = 0;
+        for (int m = 0; m < kb; ++m) g.fac[n++] = (const float*)factors[m];
+        g.fac[n++] = nullptr;  // the output digit
+        for (int m = ke; m < ndim - 2; ++m) g.fac[n++] = (const float*)factors[m];
+      }
+      g.weights = (const float*)weights;
+      const int64_t ne = p.gm_rows * R;
+      build_gm_kernel<<<ceil_div(ne, 256), 256, 0, st>>>(g, sqbuf);
+      TLA_LAUNCH_CHECK();
+    }
+    s.Sq = sqbuf;
+    s.out = (p.sepi == SE_ROWS) ? (float*)out : part;
+    if (x_dtype == TLA_BF16) rc = launch_stream<TLA_PREC_BF16, __bf16>(p, st);
+    else if (precision == TLA_PREC_FP32) rc = launch_stream<TLA_PREC_FP32, float>(p, st);
+    else if (precision == TLA_PREC_BF16) rc = launch_stream<TLA_PREC_BF16, float>(p, st);
+    else rc = launch_stream<TLA_PREC_BF16X3, float>(p, st);
+    if (rc != TLA_OK || p.sepi == SE_ROWS) return rc;
+    if (p.sepi == SE_REDQ) {
+      const int64_t n = p.t_In * R;
+      tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R, (float*)out);
+    } else {
+      const int64_t n = s.J * R;
+      accj_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.s_W, s.J, R, (float*)out);
+    }
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
+  if (p.path == Path::LastStream) {
+    LastStreamArgs& s = p.ls;
+    float* gm = (float*)ws;
+    partial = (float*)((char*)ws + align256(size_t(p.gm_rows) * R * 4));
+    GmArgs g = p.gm;
+    for (int m = 0; m < g.n; ++m) g.fac[m] = (const float*)factors[m];
+    g.weights = (const float*)weights;
+    const int64_t ne = p.gm_rows * R;
+    build_gm_kernel<<<ceil_div(ne, 256), 256, 0, st>>>(g, gm);
+    TLA_LAUNCH_CHECK();
+    s.X = X;
+    s.Fj = (const float*)factors[kb - 1];
+    s.Gm = gm;
+    s.partial = partial;
+    if (x_dtype == TLA_BF16) rc = launch_last_stream<TLA_PREC_BF16, __bf16>(p, st);
+    else if (precision == TLA_PREC_FP32) rc = launch_last_stream<TLA_PREC_FP32, float>(p, st);
+    else if (precision == TLA_PREC_BF16) rc = launch_last_stream<TLA_PREC_BF16, float>(p, st);
+    else rc = launch_last_stream<TLA_PREC_BF16X3, float>(p, st);
+    if (rc != TLA_OK) return rc;
+    const int64_t n = p.out_rows * R;
+    mttkrp_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(partial, n, p.splits, (float*)out);
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
+  if (p.path == Path::Tiles) {
+    TilesArgs& t = p.tl;
+    t.X = X;
+    for (int m = 0; m < t.na; ++m) t.afac[m] = (const float*)factors[m];
+    for (int m = 0; m < t.nb; ++m) t.bfac[m] = (const float*)factors[ke + m];
+    t.weights = (const float*)weights;
+    t.Fj = (const float*)factors[ndim - 2];
+    t.Fk = (const float*)factors[ndim - 1];
+    t.out = (p.epi == EPI_ROWS) ? (float*)out : (float*)ws;
+    if (x_dtype == TLA_BF16) rc = launch_tiles<TLA_PREC_BF16, __bf16>(p, st);
+    else if (precision == TLA_PREC_FP32) rc = launch_tiles<TLA_PREC_FP32, float>(p, st);
+    else if (precision == TLA_PREC_BF16) rc = launch_tiles<TLA_PREC_BF16, float>(p, st);
+    else rc = launch_tiles<TLA_PREC_BF16X3, float>(p, st);
+    if (rc != TLA_OK || p.epi == EPI_ROWS) return rc;
+    const int64_t n = p.t_In * R;
+    tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>((const float*)ws, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R,
+                                                          (float*)out);
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
   if (p.path == Path::Inner) {
     InnerArgs& a = p.in;
     a.X = X;

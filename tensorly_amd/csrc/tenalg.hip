@@ -411,7 +411,7 @@ extern "C" int tla_mode_dot(int dtype, const void* X, int ndim, const int64_t* s
                     (precision == TLA_PREC_FP32 || precision == TLA_PREC_BF16 || precision == TLA_PREC_BF16X3);
   if (mfma && Pb == 1 && Im % 32 == 0) {
     TtmRowsArgs a{(const float*)X, Pa, Im, (const float*)U, ldj, ldm, (int)J, (float*)out,
-                  (int)std::min<int64_t>(Im, 256)};
+                  (int)std::min<int64_t>(Im, 128)};
     if (precision == TLA_PREC_FP32) return launch_ttm_rows<TLA_PREC_FP32>(a, st);
     if (precision == TLA_PREC_BF16) return launch_ttm_rows<TLA_PREC_BF16>(a, st);
     return launch_ttm_rows<TLA_PREC_BF16X3>(a, st);

@@ -144,18 +144,22 @@ def test_tucker_matches_reference_trajectory(cuda, trajectories):
         np.testing.assert_allclose(tt.factors[m].cpu().numpy(), trajectories[f"xt_factor{m}"], atol=1e-8)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
-def test_tucker_float32_vs_oracle(cuda, prec, tol):
+@pytest.mark.parametrize("prec,tol_e2,tol_core", [("fp32", 5e-8, 1e-5), ("bf16x3", 2e-6, 1e-4)])
+def test_tucker_float32_vs_oracle(cuda, prec, tol_e2, tol_core):
+    """The reference error sqrt(|‖X‖² − ‖core‖²|)/‖X‖ cancels: at 1 % noise
+    err² ≈ 1e-4, so the check is on err² (i.e. on ‖core‖²/‖X‖²), with the
+    contraction's own relative accuracy as tolerance."""
     import tensorly_amd as tla
     x, _ = synthetic.tucker_256_core32(dtype=np.float32, size=96, rank=12)
     (core, fac), ref_errs = T.tucker(x.astype(np.float64), rank=[12, 12, 12], n_iter_max=5, tol=0,
                                      return_errors=True)
     tt, errs = tla.tucker(g(x, cuda, torch.float32), rank=[12, 12, 12], n_iter_max=5, tol=0, return_errors=True,
                           precision=prec)
-    np.testing.assert_allclose(errs, ref_errs, rtol=tol)
+    d = np.abs(np.array(errs) ** 2 - np.array(ref_errs) ** 2)
+    assert np.max(d) < tol_e2, d
     for m in range(3):
         # principal angles between subspaces
         s = np.linalg.svd(fac[m].T @ tt.factors[m].cpu().numpy().astype(np.float64), compute_uv=False)
         assert np.min(s) > 1 - 1e-6
-    # the core in the reference's basis (sign rule makes the bases agree)
-    assert rel(tt.core.cpu().numpy(), core) < 10 * tol
+    # the core in the reference's basis (the sign rule makes the bases agree)
+    assert rel(tt.core.cpu().numpy(), core) < tol_core

@@ -1,6 +1,8 @@
 """HIP tenalg kernels against the oracle (same seeded inputs), through the
 C ABI.  Tolerances: FP64 paths 1e-12 relative; FP32 and BF16x3 MTTKRP 1e-5
 relative (Frobenius) against the FP64 oracle; plain BF16 1e-2."""
+import zlib
+
 import numpy as np
 import pytest
 import torch
@@ -47,14 +49,15 @@ def test_khatri_rao_weights_mask_float32(cuda):
     assert rel(out.cpu(), ref) < 1e-6
 
 
-SHAPES = [(20, 24, 32), (16, 32, 64), (7, 9, 11), (12, 10, 8, 32), (5, 6, 7, 8)]
+SHAPES = [(20, 24, 32), (16, 32, 64), (7, 9, 11), (12, 10, 8, 32), (5, 6, 7, 8), (40, 48, 128),
+          (8, 16, 32, 64)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("rank", [1, 5, 16, 32, 64])
 def test_mttkrp_fp64_valu(cuda, shape, rank):
     import tensorly_amd as tla
-    rng = np.random.default_rng(hash((shape, rank)) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, rank)).encode()))
     X = rng.standard_normal(shape)
     w = rng.random(rank) + 0.5
     f = [rng.standard_normal((s, rank)) for s in shape]
@@ -71,7 +74,7 @@ def test_mttkrp_fp64_valu(cuda, shape, rank):
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-5), ("bf16", 1e-2), ("valu", 1e-5)])
 def test_mttkrp_float32(cuda, shape, rank, prec, tol):
     import tensorly_amd as tla
-    rng = np.random.default_rng(abs(hash((shape, rank, prec))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, rank, prec)).encode()))
     X = rng.random(shape).astype(np.float32)
     w = (rng.random(rank) + 0.5).astype(np.float32)
     f = [rng.random((s, rank)).astype(np.float32) for s in shape]
@@ -129,7 +132,7 @@ def test_mttkrp_deterministic(cuda):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("shape", [(16, 24, 32), (8, 16, 8, 32)])
+@pytest.mark.parametrize("shape", [(16, 24, 32), (8, 16, 8, 32), (16, 24, 64), (8, 16, 16, 64)])
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_partial_mttkrp_and_rankdiag(cuda, shape, prec):
     import tensorly_amd as tla
@@ -171,7 +174,7 @@ def test_mode_dot_known_answers(cuda, known_answers):
 @pytest.mark.parametrize("transpose", [False, True])
 def test_mode_dot_float32(cuda, shape, J, prec, tol, transpose):
     import tensorly_amd as tla
-    rng = np.random.default_rng(abs(hash((shape, J, prec, transpose))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, J, prec, transpose)).encode()))
     X = rng.standard_normal(shape).astype(np.float32)
     with tla.precision(prec):
         for mode in range(len(shape)):
@@ -195,7 +198,7 @@ def test_multi_mode_dot_semantics(cuda):
     assert rel(tla.multi_mode_dot(Xg, fg, skip=1).cpu(), ref) < 1e-12
     ref = T.multi_mode_dot(X, [f.T for f in factors], transpose=True)
     assert rel(tla.multi_mode_dot(Xg, [f.T.contiguous() for f in fg], transpose=True).cpu(), ref) < 1e-12
-    vecs = [rng.random(s) for s in dims]
+    vecs = [rng.random(dims[m]) for m in (3, 1, 0, 2)]
     ref = T.multi_mode_dot(X, vecs, modes=[3, 1, 0, 2])
     out = tla.multi_mode_dot(Xg, [g(v, cuda) for v in vecs], modes=[3, 1, 0, 2])
     assert out.dim() == 0
