@@ -76,6 +76,9 @@ int tla_mttkrp(int x_dtype, const void* X, int ndim, const int64_t* shape,
 size_t tla_mttkrp_partial_workspace_size(int x_dtype, int ndim, const int64_t* shape,
                                          int rank, int keep_begin, int keep_end,
                                          int precision);
+/* 1 if the partial contraction has a kernel for this shape / precision */
+int tla_mttkrp_partial_supported(int x_dtype, int ndim, const int64_t* shape, int rank,
+                                 int keep_begin, int keep_end, int precision);
 int tla_mttkrp_partial(int x_dtype, const void* X, int ndim, const int64_t* shape,
                        const void* const* factors, const void* weights, int rank,
                        int keep_begin, int keep_end, int precision, void* out,

@@ -84,9 +84,8 @@ struct Layout {
 static bool dimtree_ok(int x_dtype, int ndim, const int64_t* shape, int R, int precision) {
   if (!(ndim == 3 || ndim == 4)) return false;
   if (x_dtype == TLA_F64 || precision == TLA_PREC_VALU) return false;
-  if (tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 0, 2, precision) == 0) return false;
-  if (ndim == 4 && tla_mttkrp_partial_workspace_size(x_dtype, ndim, shape, R, 2, 4, precision) == 0)
-    return false;
+  if (!tla_mttkrp_partial_supported(x_dtype, ndim, shape, R, 0, 2, precision)) return false;
+  if (ndim == 4 && !tla_mttkrp_partial_supported(x_dtype, ndim, shape, R, 2, 4, precision)) return false;
   return true;
 }
 

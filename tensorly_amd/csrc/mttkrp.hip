@@ -1697,3 +1697,11 @@ extern "C" int tla_mttkrp_partial(int x_dtype, const void* X, int ndim, const in
   return run_plan(p, x_dtype, X, ndim, shape, factors, weights, rank, keep_begin, keep_end,
                   precision, out, workspace, workspace_bytes, (hipStream_t)stream);
 }
+
+extern "C" int tla_mttkrp_partial_supported(int x_dtype, int ndim, const int64_t* shape, int rank, int keep_begin,
+                                            int keep_end, int precision) {
+  if (!(keep_begin == 0 || keep_end == ndim) || (keep_begin == 0 && keep_end == ndim)) return 0;
+  Plan p;
+  if (make_plan(x_dtype, ndim, shape, rank, keep_begin, keep_end, precision, &p) != TLA_OK) return 0;
+  return p.path != Path::Generic ? 1 : 0;
+}
