@@ -131,12 +131,15 @@ int tla_gram64(int dtype, const void* A, int64_t rows, int rank, double* G, void
  * _cp.py:222).  grams: host array of ndim device pointers to R x R FP64.
  * Returns TLA_ERR_NUMERIC only through `status_out` (device int, nullable):
  * 0 ok, 1 LU fallback used, 2 singular.
+ * `counter` (device int, nullable) must hold 0 on entry and is left at 0: the
+ * last workgroup then sums the per-block partials itself (one launch instead
+ * of two); NULL uses a separate finalising launch.
  * ------------------------------------------------------------------- */
 size_t tla_cp_solve_workspace_size(int64_t rows, int rank);
 int tla_cp_solve(int dtype, const double* const* grams, int ndim, int mode,
                  const void* weights, double l2_reg, const void* mttkrp, int64_t rows, int rank,
                  void* factor_out, double* gram_out, double* iprod_out, int* status_out,
-                 void* workspace, size_t workspace_bytes, void* stream);
+                 int* counter, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Squared Frobenius norm with FP64 accumulation (the norm of
@@ -209,6 +212,44 @@ int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int first_slot
  * 2 last <M, A> (FP64), 3 last solve status (int) */
 int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t bytes, void* stream);
 void tla_cp_als_destroy(void* handle);
+
+/* ---------------------------------------------------------------------
+ * Learned extrapolation of the ALS iterates.  Replaces the fixed step
+ * jump = iteration^(1/acc_pow) of the reference line search
+ * (tensorly/decomposition/_cp.py:470-503) by a step chosen per iteration by
+ * a 16 -> 64 -> 64 -> 1 tanh network whose input is the error history.
+ *   tla_mlp_num_params: parameter count (5313), packed
+ *     W1[64][16] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3[1]  (FP32).
+ *   tla_mlp_forward:  out[i] = net(X[i, 0:16]) for i < n.
+ *   tla_mlp_train:    `steps` Adam steps of mean squared error on minibatches
+ *     of `batch` (<= 128) rows taken from `order` (device int, n entries,
+ *     used cyclically); Adam moments in adam_m/adam_v, step count starts at
+ *     t0 + 1; loss[step] (device, nullable) receives each minibatch loss.
+ *     With grad_out non-NULL no update is made: one step over the first
+ *     `batch` rows writes the exact parameter gradient there.
+ *   tla_cp_als_run_extrapolated: n_sweeps sweeps of the driver with error
+ *     slots errors[0..n_sweeps).  After `warmup` plain sweeps it repeats:
+ *     save A_prev, one sweep, alpha = clip(2^net(features) - 1, 0, alpha_max)
+ *     (alpha_fixed when mlp_params is NULL), A <- A + alpha (A - A_prev), one
+ *     sweep from the extrapolated point, and keep it only if the error fell
+ *     (accepted[s] = 1), else restore the pre-extrapolation factors and copy
+ *     the previous error forward.  alphas[s], accepted[s] (device) describe
+ *     slot s; features (nullable, n_sweeps x 16 FP32) receives the network
+ *     inputs.  Workspace: tla_cp_als_extrap_workspace_size(handle) bytes.
+ *   tla_extrap_features: the 16 network inputs of slot s from the error,
+ *     alpha and acceptance history (the same device code the driver runs).
+ * ------------------------------------------------------------------- */
+int tla_mlp_num_params(void);
+int tla_mlp_forward(const float* params, const float* X, int n, float* out, void* stream);
+int tla_mlp_train(float* params, float* adam_m, float* adam_v, const float* X, const float* y, const int* order,
+                  int n, int batch, int steps, float lr, float beta1, float beta2, float eps, int t0, float* loss,
+                  float* grad_out, void* stream);
+size_t tla_cp_als_extrap_workspace_size(void* handle);
+int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const float* mlp_params, float alpha_fixed,
+                                float alpha_max, int warmup, double* errors, int* accepted, float* alphas,
+                                float* features, void* workspace, size_t workspace_bytes, void* stream);
+int tla_extrap_features(const double* errors, const float* alphas, const int* accepted, int s, float* out,
+                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ c_i = ctypes.c_int
 c_i64 = ctypes.c_int64
 c_sz = ctypes.c_size_t
 c_d = ctypes.c_double
+c_f = ctypes.c_float
 P_i64 = ctypes.POINTER(ctypes.c_int64)
 PP = ctypes.POINTER(ctypes.c_void_p)
 
@@ -47,7 +48,7 @@ _SIGNATURES = {
     "tla_gram64_workspace_size": (c_sz, [c_i64, c_i]),
     "tla_gram64": (c_i, [c_i, c_p, c_i64, c_i, c_p, c_p, c_sz, c_p]),
     "tla_cp_solve_workspace_size": (c_sz, [c_i64, c_i]),
-    "tla_cp_solve": (c_i, [c_i, PP, c_i, c_i, c_p, c_d, c_p, c_i64, c_i, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    "tla_cp_solve": (c_i, [c_i, PP, c_i, c_i, c_p, c_d, c_p, c_i64, c_i, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "tla_sqnorm_workspace_size": (c_sz, [c_i64]),
     "tla_sqnorm": (c_i, [c_i, c_p, c_i64, c_p, c_p, c_sz, c_p]),
     "tla_cp_error": (c_i, [PP, c_i, c_p, c_i, c_i, c_p, c_p, c_p, c_i, c_p]),
@@ -61,6 +62,12 @@ _SIGNATURES = {
     "tla_cp_als_sweeps": (c_i, [c_p, c_i, c_p, c_i, c_i, c_p]),
     "tla_cp_als_copy_state": (c_i, [c_p, c_i, c_p, c_sz, c_p]),
     "tla_cp_als_destroy": (None, [c_p]),
+    "tla_mlp_num_params": (c_i, []),
+    "tla_mlp_forward": (c_i, [c_p, c_p, c_i, c_p, c_p]),
+    "tla_mlp_train": (c_i, [c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i, c_i, c_f, c_f, c_f, c_f, c_i, c_p, c_p, c_p]),
+    "tla_cp_als_extrap_workspace_size": (c_sz, [c_p]),
+    "tla_cp_als_run_extrapolated": (c_i, [c_p, c_i, c_p, c_f, c_f, c_i, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    "tla_extrap_features": (c_i, [c_p, c_p, c_p, c_i, c_p, c_p]),
 }
 
 

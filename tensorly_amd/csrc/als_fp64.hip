@@ -102,6 +102,9 @@ struct SolveArgs {
   double* gram_partial;  // [nblk][R][R]
   double* ip_partial;    // [nblk] or nullptr
   int* status;           // nullable
+  int* counter;          // nullable: zero-initialised block counter -> the last block finalises
+  double* gram_out;
+  double* iprod_out;
 };
 
 // In-place Cholesky of the R x R matrix in LDS (lower triangle).  Returns
@@ -218,8 +221,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __shared__ double Lbuf[kMaxRank * kMaxRank];
   __shared__ double Vi[kMaxRank * kMaxRank];
-  __shared__ int flag, piv;
+  __shared__ double Mr[kSolveRows * kMaxRank];
+  __shared__ int flag, piv, ticket;
   const int R = a.R, tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
+  const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
+  // this block's MTTKRP rows, staged once (coalesced)
+  for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
   build_v<T>(a, Lbuf);
   int status = 0;
   if (!lds_cholesky(Lbuf, R, &flag)) {
@@ -232,26 +240,34 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __syncthreads();
   // New rows: A[i, s] = sum_r M[i, r] Vi[r, s]  (V symmetric); keep the
   // rounded values in LDS (reusing Lbuf) for the Gram / inner product.
-  const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
-  const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
   double* An = Lbuf;  // kSolveRows x R
   double ip = 0.0;
   for (int e = tid; e < nr * R; e += blockDim.x) {
     const int i = e / R, s = e % R;
-    const int64_t gi = r0 + i;
-    double v = 0.0;
-    for (int r = 0; r < R; ++r) v = fma(ld<T>(a.mttkrp, gi * R + r), Vi[r * R + s], v);
-    const T out = (T)v;
-    ((T*)a.factor)[gi * R + s] = out;
+    const double* mrow = Mr + i * R;
+    double v0 = 0.0, v1 = 0.0;
+    int r = 0;
+    for (; r + 1 < R; r += 2) {
+      v0 = fma(mrow[r], Vi[r * R + s], v0);
+      v1 = fma(mrow[r + 1], Vi[(r + 1) * R + s], v1);
+    }
+    if (r < R) v0 = fma(mrow[r], Vi[r * R + s], v0);
+    const T out = (T)(v0 + v1);
+    ((T*)a.factor)[(r0 + i) * R + s] = out;
     An[i * R + s] = (double)out;
-    ip = fma(ld<T>(a.mttkrp, gi * R + s), (double)out, ip);
+    ip = fma(mrow[s], (double)out, ip);
   }
   __syncthreads();
   for (int e = tid; e < R * R; e += blockDim.x) {
     const int r = e / R, s = e % R;
-    double acc = 0.0;
-    for (int i = 0; i < nr; ++i) acc = fma(An[i * R + r], An[i * R + s], acc);
-    a.gram_partial[(int64_t)blockIdx.x * R * R + e] = acc;
+    double acc0 = 0.0, acc1 = 0.0;
+    int i = 0;
+    for (; i + 1 < nr; i += 2) {
+      acc0 = fma(An[i * R + r], An[i * R + s], acc0);
+      acc1 = fma(An[(i + 1) * R + r], An[(i + 1) * R + s], acc1);
+    }
+    if (i < nr) acc0 = fma(An[i * R + r], An[i * R + s], acc0);
+    a.gram_partial[(int64_t)blockIdx.x * R * R + e] = acc0 + acc1;
   }
   if (a.ip_partial) {
     ip = wave_sum(ip);
@@ -260,6 +276,27 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
     __syncthreads();
     if (tid == 0) a.ip_partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
   }
+  if (!a.counter) return;
+  // The last block to finish sums every block's partials in block order.
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) ticket = atomicAdd(a.counter, 1);
+  __syncthreads();
+  if (ticket != (int)gridDim.x - 1) return;
+  __threadfence();
+  const int nblk = gridDim.x;
+  for (int e = tid; e < R * R; e += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b)
+      s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a.gram_out[e] = s;
+  }
+  if (a.ip_partial && tid == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += __hip_atomic_load(a.ip_partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *a.iprod_out = s;
+  }
+  if (tid == 0) *a.counter = 0;
 }
 
 // err = sqrt(|normx2 + sum((prod G) * w w^T) - 2 iprod|); errors[slot] = err / sqrt(normx2)
@@ -347,7 +384,7 @@ extern "C" size_t tla_cp_solve_workspace_size(int64_t rows, int rank) {
 extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int mode,
                             const void* weights, double l2_reg, const void* mttkrp, int64_t rows,
                             int rank, void* factor_out, double* gram_out, double* iprod_out,
-                            int* status_out, void* ws, size_t ws_bytes, void* stream) {
+                            int* status_out, int* counter, void* ws, size_t ws_bytes, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_solve: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_solve: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_solve: mode %d", mode);
@@ -366,12 +403,17 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
   a.gram_partial = (double*)ws;
   a.ip_partial = iprod_out ? (double*)((char*)ws + align256(size_t(nblk) * rank * rank * 8)) : nullptr;
   a.status = status_out;
+  a.counter = counter;
+  a.gram_out = gram_out;
+  a.iprod_out = iprod_out;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TLA_F32) cp_solve_kernel<float><<<nblk, 256, 0, st>>>(a);
   else cp_solve_kernel<double><<<nblk, 256, 0, st>>>(a);
   TLA_LAUNCH_CHECK();
-  gram_finalize_kernel<<<1, 256, 0, st>>>(a.gram_partial, nblk, rank, gram_out, a.ip_partial, iprod_out);
-  TLA_LAUNCH_CHECK();
+  if (!counter) {
+    gram_finalize_kernel<<<1, 256, 0, st>>>(a.gram_partial, nblk, rank, gram_out, a.ip_partial, iprod_out);
+    TLA_LAUNCH_CHECK();
+  }
   return TLA_OK;
 }
 

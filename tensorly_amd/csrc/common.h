@@ -69,6 +69,18 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// out[g * E + e] = sum_{k < K} in[g * gs + k * ks + e]  for e < E, g < gridDim.y.
+// Block = 4 k-lanes x 64 elements; each lane keeps four independent partial
+// sums (so loads overlap), combined in a fixed order: deterministic.
+__global__ void __launch_bounds__(256) strided_reduce_kernel(const float* __restrict__ in, int64_t E, int64_t K,
+                                                             int64_t gs, int64_t ks, float* __restrict__ out);
+
+inline void strided_reduce(const float* in, int64_t groups, int64_t E, int64_t K, int64_t gs, int64_t ks, float* out,
+                           hipStream_t st) {
+  dim3 grid((unsigned)ceil_div(E, 64), (unsigned)groups);
+  strided_reduce_kernel<<<grid, 256, 0, st>>>(in, E, K, gs, ks, out);
+}
+
 struct ModeShape {
   int ndim;
   int64_t shape[kMaxOrder];

@@ -46,6 +46,10 @@ __global__ void normalize_scales_kernel(GramPtrs grams, int ndim, int R, void* w
   w[r] = (float)neww;
 }
 
+__global__ void zero_ints_kernel(int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 __global__ void scale_columns_kernel(float* A, int64_t rows, int R, const double* f) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= rows * R) return;
@@ -66,6 +70,7 @@ struct CpAls {
   double* iprod;
   double* scales;
   int* status;
+  int* counter;
   float* M;
   float* T1;
   float* T2;
@@ -134,7 +139,7 @@ static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, hipStrea
   for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
   return tla_cp_solve(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n],
-                      h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->ws, h->ws_bytes, st);
+                      h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws, h->ws_bytes, st);
 }
 
 static int mttkrp_mode(CpAls* h, int n, hipStream_t st) {
@@ -249,6 +254,7 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   h->iprod = (double*)(base + L.iprod);
   h->scales = (double*)(base + L.scales);
   h->status = (int*)(base + L.status);
+  h->counter = h->status + 1;  // the status slot is 8 bytes: int status, int block counter
   h->M = (float*)(base + L.M);
   h->T1 = dimtree ? (float*)(base + L.T1) : nullptr;
   h->T2 = (dimtree && ndim == 4) ? (float*)(base + L.T2) : nullptr;
@@ -270,6 +276,8 @@ extern "C" int tla_cp_als_refresh(void* handle, int with_norm, void* stream) {
   TLA_REQUIRE(h, "cp_als: NULL handle");
   hipStream_t st = (hipStream_t)stream;
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  zero_ints_kernel<<<1, 64, 0, st>>>(h->status, 2);
+  TLA_LAUNCH_CHECK();
   if (with_norm) {
     int64_t total = 1;
     for (int m = 0; m < h->ndim; ++m) total *= h->shape[m];
@@ -346,5 +354,213 @@ extern "C" int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t 
   }
   TLA_REQUIRE(bytes <= avail, "cp_als: %zu bytes requested, state %d holds %zu", bytes, which, avail);
   TLA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return TLA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Extrapolated ALS (learned step).  Every second sweep the factors are moved
+// along the last plain sweep's update, A + alpha (A - A_prev), and the next
+// sweep runs from there; the step is kept only if that sweep's error is below
+// the error before it, otherwise the saved factors are restored (the sweep
+// was wasted).  alpha comes from the MLP over 16 features of the error
+// history (or a fixed value).  All decisions are taken on the device.
+// ---------------------------------------------------------------------------
+namespace tla {
+
+constexpr int kMlpIn = 16, kMlpH = 64;
+constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;
+
+struct FactorPtrs {
+  float* p[kMaxOrder];
+  int64_t n[kMaxOrder];  // elements
+};
+
+__device__ void extrap_features(const double* errors, const float* alphas, const int* accepted, int s, float* f) {
+  double e[5];
+  for (int k = 0; k < 5; ++k) e[k] = errors[max(s - k, 0)];
+  double d[4];
+  for (int k = 0; k < 4; ++k) d[k] = e[k + 1] - e[k];
+  const double e0 = fmax(e[0], 1e-30);
+  f[0] = (float)log10(e0);
+  for (int k = 0; k < 4; ++k) f[1 + k] = (float)log10(fmax(fabs(d[k]), 1e-30) / e0);
+  for (int k = 0; k < 3; ++k) f[5 + k] = (d[k + 1] != 0.0) ? (float)fmin(fmax(d[k] / d[k + 1], -2.0), 2.0) : 0.0f;
+  const float a1 = (s >= 1) ? alphas[s - 1] : 0.0f;
+  const float a2 = (s >= 3) ? alphas[s - 3] : 0.0f;
+  f[8] = tanhf(a1 / 4.0f);
+  f[9] = tanhf(a2 / 4.0f);
+  f[10] = (s >= 1) ? (float)accepted[s - 1] : 0.5f;
+  float acc = 0.0f;
+  int cnt = 0;
+  for (int k = 1; k <= 7; k += 2)
+    if (s - k >= 0) { acc += (float)accepted[s - k]; ++cnt; }
+  f[11] = cnt ? acc / cnt : 0.5f;
+  f[12] = (float)(log10((double)s + 1.0) / 3.0);
+  f[13] = (d[0] > 0.0) ? 1.0f : ((d[0] < 0.0) ? -1.0f : 0.0f);
+  f[14] = (float)(fmin(fmax((e[4] - e[0]) / (4.0 * fmax(fabs(d[0]), 1e-30)), -4.0), 4.0) / 4.0);
+  f[15] = 1.0f;
+}
+
+// One block: features of sweep s, MLP forward, alpha -> alphas[s + 1].
+__global__ void extrap_alpha_kernel(const double* errors, float* alphas, const int* accepted, int s,
+                                    const float* params, float alpha_fixed, float alpha_max, float* feat_out,
+                                    int write_alpha) {
+  __shared__ float f[kMlpIn];
+  __shared__ float h1[kMlpH];
+  __shared__ float part[kMlpH];
+  const int t = threadIdx.x;
+  if (t == 0) extrap_features(errors, alphas, accepted, s, f);
+  __syncthreads();
+  if (feat_out && t < kMlpIn) feat_out[(int64_t)s * kMlpIn + t] = f[t];
+  if (!write_alpha) return;
+  if (!params) {
+    if (t == 0) alphas[s + 1] = alpha_fixed;
+    return;
+  }
+  const float* W1 = params;
+  const float* b1 = W1 + kMlpH * kMlpIn;
+  const float* W2 = b1 + kMlpH;
+  const float* b2 = W2 + kMlpH * kMlpH;
+  const float* w3 = b2 + kMlpH;
+  const float* b3 = w3 + kMlpH;
+  if (t < kMlpH) {
+    float z = b1[t];
+    for (int i = 0; i < kMlpIn; ++i) z = fmaf(W1[t * kMlpIn + i], f[i], z);
+    h1[t] = tanhf(z);
+  }
+  __syncthreads();
+  if (t < kMlpH) {
+    float z = b2[t];
+    for (int j = 0; j < kMlpH; ++j) z = fmaf(W2[t * kMlpH + j], h1[j], z);
+    part[t] = w3[t] * tanhf(z);
+  }
+  __syncthreads();
+  if (t == 0) {
+    float y = b3[0];
+    for (int k = 0; k < kMlpH; ++k) y += part[k];
+    float a = exp2f(y) - 1.0f;
+    a = fminf(fmaxf(a, 0.0f), alpha_max);
+    if (!(a == a)) a = 0.0f;
+    alphas[s + 1] = a;
+  }
+}
+
+// A_save = A; A = A + alpha (A - A_prev)   (blockIdx.y = factor)
+__global__ void extrapolate_kernel(FactorPtrs A, FactorPtrs Aprev, FactorPtrs Asave, const float* alphas, int slot) {
+  const int m = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.n[m]) return;
+  const float a = alphas[slot];
+  const float x = A.p[m][e];
+  Asave.p[m][e] = x;
+  A.p[m][e] = x + a * (x - Aprev.p[m][e]);
+}
+
+__global__ void copy_factors_kernel(FactorPtrs dst, FactorPtrs src) {
+  const int m = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= src.n[m]) return;
+  dst.p[m][e] = src.p[m][e];
+}
+
+// keep the extrapolated step iff errors[s + 1] < errors[s]; else restore
+__global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, double* errors, int* accepted, int s) {
+  const bool ok = errors[s + 1] < errors[s];
+  const int m = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!ok && e < A.n[m]) A.p[m][e] = Asave.p[m][e];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    accepted[s + 1] = ok ? 1 : 0;
+    if (!ok) errors[s + 1] = errors[s];
+  }
+}
+
+}  // namespace tla
+
+extern "C" int tla_mlp_num_params(void);
+
+extern "C" size_t tla_cp_als_extrap_workspace_size(void* handle) {
+  CpAls* h = (CpAls*)handle;
+  if (!h) return 0;
+  size_t tot = 0;
+  for (int m = 0; m < h->ndim; ++m) tot += align256(size_t(h->shape[m]) * h->R * 4);
+  return 2 * tot;
+}
+
+extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const float* mlp_params, float alpha_fixed,
+                                           float alpha_max, int warmup, double* errors, int* accepted, float* alphas,
+                                           float* features, void* ws, size_t ws_bytes, void* stream) {
+  CpAls* h = (CpAls*)handle;
+  TLA_REQUIRE(h && errors && accepted && alphas, "cp_als_extrapolated: NULL pointer");
+  TLA_REQUIRE(h->initialised, "cp_als: call tla_cp_als_refresh first");
+  TLA_REQUIRE(h->x_dtype != TLA_F64, "cp_als_extrapolated: float32 factors only");
+  TLA_REQUIRE(ws_bytes >= tla_cp_als_extrap_workspace_size(handle), "cp_als_extrapolated: workspace too small");
+  TLA_REQUIRE(warmup >= 1, "cp_als_extrapolated: warmup must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  const double* gp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
+  FactorPtrs A{}, Aprev{}, Asave{};
+  char* base = (char*)ws;
+  size_t off = 0;
+  int64_t maxn = 0;
+  for (int m = 0; m < h->ndim; ++m) {
+    const int64_t n = h->shape[m] * h->R;
+    maxn = std::max(maxn, n);
+    A.p[m] = (float*)h->factors[m];
+    A.n[m] = Aprev.n[m] = Asave.n[m] = n;
+    Aprev.p[m] = (float*)(base + off);
+    off += align256(size_t(n) * 4);
+  }
+  for (int m = 0; m < h->ndim; ++m) {
+    Asave.p[m] = (float*)(base + off);
+    off += align256(size_t(A.n[m]) * 4);
+  }
+  const dim3 fgrid((unsigned)ceil_div(maxn, 256), (unsigned)h->ndim);
+  auto sweep = [&](int slot) -> int {
+    int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
+    if (rc) return rc;
+    return tla_cp_error(gp, h->ndim, h->weights, TLA_F32, h->R, h->normx2, h->iprod, errors, slot, st);
+  };
+  auto refresh = [&]() -> int {
+    for (int m = 0; m < h->ndim; ++m) {
+      int rc = tla_gram64(TLA_F32, h->factors[m], h->shape[m], h->R, h->grams[m], h->ws, h->ws_bytes, st);
+      if (rc) return rc;
+    }
+    return TLA_OK;
+  };
+  int s = 0;
+  // plain warm-up sweeps (flags/alphas for these slots are set by the caller)
+  for (; s < std::min(warmup, n_sweeps); ++s) {
+    int rc = sweep(s);
+    if (rc) return rc;
+  }
+  while (s < n_sweeps) {
+    copy_factors_kernel<<<fgrid, 256, 0, st>>>(Aprev, A);
+    TLA_LAUNCH_CHECK();
+    int rc = sweep(s);
+    if (rc) return rc;
+    if (s + 1 >= n_sweeps) break;
+    extrap_alpha_kernel<<<1, 64, 0, st>>>(errors, alphas, accepted, s, mlp_params, alpha_fixed, alpha_max,
+                                           features, 1);
+    TLA_LAUNCH_CHECK();
+    extrapolate_kernel<<<fgrid, 256, 0, st>>>(A, Aprev, Asave, alphas, s + 1);
+    TLA_LAUNCH_CHECK();
+    if ((rc = refresh())) return rc;
+    if ((rc = sweep(s + 1))) return rc;
+    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, errors, accepted, s);
+    TLA_LAUNCH_CHECK();
+    if ((rc = refresh())) return rc;
+    s += 2;
+  }
+  return TLA_OK;
+}
+
+// Features of sweep s (the network's input) written to out[s * 16 ..], so
+// training data uses exactly the inference-time code.
+extern "C" int tla_extrap_features(const double* errors, const float* alphas, const int* accepted, int s,
+                                   float* out, void* stream) {
+  TLA_REQUIRE(errors && alphas && accepted && out && s >= 0, "extrap_features: bad argument");
+  tla::extrap_alpha_kernel<<<1, 64, 0, (hipStream_t)stream>>>(errors, const_cast<float*>(alphas), accepted, s,
+                                                               nullptr, 0.0f, 0.0f, out, 0);
+  TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -1557,10 +1557,15 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     if (rc != TLA_OK || p.sepi == SE_ROWS) return rc;
     if (p.sepi == SE_REDQ) {
       const int64_t n = p.t_In * R;
-      tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R, (float*)out);
+      if (p.t_Pa == 1)
+        strided_reduce(part, p.t_In, R, p.t_Pb * p.t_S, p.t_Pb * p.t_S * R, R, (float*)out, st);
+      else
+        tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R, (float*)out);
     } else {
       const int64_t n = s.J * R;
-      accj_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.s_W, s.J, R, (float*)out);
+      (void)n;
+      const int64_t tpq = s.J / 16;
+      strided_reduce(part, tpq, 16 * R, p.s_W / tpq, 16 * R, tpq * 16 * R, (float*)out, st);
     }
     TLA_LAUNCH_CHECK();
     return TLA_OK;
@@ -1585,7 +1590,7 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     else rc = launch_last_stream<TLA_PREC_BF16X3, float>(p, st);
     if (rc != TLA_OK) return rc;
     const int64_t n = p.out_rows * R;
-    mttkrp_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(partial, n, p.splits, (float*)out);
+    strided_reduce(partial, 1, n, p.splits, 0, n, (float*)out, st);
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
@@ -1642,7 +1647,7 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
   }
   if (rc != TLA_OK) return rc;
   const int64_t n = p.out_rows * R;
-  mttkrp_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(partial, n, p.splits, (float*)out);
+  strided_reduce(partial, 1, n, p.splits, 0, n, (float*)out, st);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -283,39 +283,61 @@ __global__ __launch_bounds__(256) void ttm_cols_kernel(TtmColsArgs a) {
 // ---------------------------------------------------------------------------
 // Rank-diagonal contraction of Y (I x J x R)
 // ---------------------------------------------------------------------------
-// axis 1: out[i, r] = sum_j Y[i, j, r] F[j, r]  -- one block per i
+// axis 1: out[i, r] = sum_j Y[i, j, r] F[j, r]  -- one block per i; lane
+// group jl of L = 256 / R sums j = jl, jl + L, ... with four independent
+// partial sums, then the L groups are combined in order.
 __global__ __launch_bounds__(256) void rankdiag_axis1_kernel(const float* __restrict__ Y, int64_t J, int R,
-                                                             const float* __restrict__ F,
-                                                             float* __restrict__ out) {
+                                                             const float* __restrict__ F, float* __restrict__ out) {
   const int64_t i = blockIdx.x;
   __shared__ float red[256];
-  // thread t handles r = t % R over j = t / R, t / R + (256 / R), ...
-  const int lanes_per_r = 256 / R;
-  const int r = threadIdx.x % R, js = threadIdx.x / R;
-  float s = 0.0f;
-  if (js < lanes_per_r)
-    for (int64_t j = js; j < J; j += lanes_per_r) s += Y[(i * J + j) * R + r] * F[j * R + r];
-  red[threadIdx.x] = s;
+  const int L = 256 / R;
+  const int r = threadIdx.x % R, jl = threadIdx.x / R;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (jl < L) {
+    const float* y = Y + i * J * R + r;
+    const float* f = F + r;
+    int64_t j = jl;
+    for (; j + 3 * L < J; j += 4 * L) {
+      s0 = fmaf(y[j * R], f[j * R], s0);
+      s1 = fmaf(y[(j + L) * R], f[(j + L) * R], s1);
+      s2 = fmaf(y[(j + 2 * L) * R], f[(j + 2 * L) * R], s2);
+      s3 = fmaf(y[(j + 3 * L) * R], f[(j + 3 * L) * R], s3);
+    }
+    for (; j < J; j += L) s0 = fmaf(y[j * R], f[j * R], s0);
+  }
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   if (threadIdx.x < R) {
     float t = 0.0f;
-    for (int k = 0; k < lanes_per_r; ++k) t += red[k * R + threadIdx.x];
+    for (int k = 0; k < L; ++k) t += red[k * R + threadIdx.x];
     out[i * R + threadIdx.x] = t;
   }
 }
 
-// axis 0: out[j, r] = sum_i Y[i, j, r] F[i, r] -- partials over i-chunks
-__global__ __launch_bounds__(256) void rankdiag_axis0_kernel(const float* __restrict__ Y, int64_t I, int64_t J,
-                                                             int R, const float* __restrict__ F,
-                                                             int64_t i_per, float* __restrict__ partial) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over J*R
-  if (e >= J * R) return;
-  const int r = (int)(e % R);
-  const int64_t i0 = (int64_t)blockIdx.y * i_per;
-  const int64_t i1 = min(I, i0 + i_per);
-  float s = 0.0f;
-  for (int64_t i = i0; i < i1; ++i) s += Y[i * J * R + e] * F[i * R + r];
-  partial[(int64_t)blockIdx.y * J * R + e] = s;
+// axis 0: out[j, r] = sum_i Y[i, j, r] F[i, r] -- block = 4 i-lanes x 64
+// consecutive (j, r) elements, every lane over a quarter of i, combined in
+// order (single launch, no workspace).
+__global__ __launch_bounds__(256) void rankdiag_axis0_kernel(const float* __restrict__ Y, int64_t I, int64_t J, int R,
+                                                             const float* __restrict__ F, float* __restrict__ out) {
+  __shared__ float red[4][64];
+  const int el = threadIdx.x & 63, il = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + el;
+  const int64_t JR = J * R;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (e < JR) {
+    const int r = (int)(e % R);
+    int64_t i = il;
+    for (; i + 12 < I; i += 16) {
+      s0 = fmaf(Y[i * JR + e], F[i * R + r], s0);
+      s1 = fmaf(Y[(i + 4) * JR + e], F[(i + 4) * R + r], s1);
+      s2 = fmaf(Y[(i + 8) * JR + e], F[(i + 8) * R + r], s2);
+      s3 = fmaf(Y[(i + 12) * JR + e], F[(i + 12) * R + r], s3);
+    }
+    for (; i < I; i += 4) s0 = fmaf(Y[i * JR + e], F[i * R + r], s0);
+  }
+  red[il][el] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (il == 0 && e < JR) out[e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
 }
 
 __global__ void sum_splits_kernel(const float* __restrict__ partial, int64_t n, int splits,
@@ -433,9 +455,8 @@ extern "C" int tla_mode_dot(int dtype, const void* X, int ndim, const int64_t* s
 }
 
 extern "C" size_t tla_rankdiag_contract_workspace_size(int64_t I, int64_t J, int rank, int axis) {
-  if (axis == 1) return 0;
-  const int64_t splits = std::min<int64_t>(I, 64);
-  return align256(size_t(splits) * J * rank * 4);
+  (void)I; (void)J; (void)rank; (void)axis;
+  return 0;
 }
 
 extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const float* F, int axis,
@@ -450,13 +471,8 @@ extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int r
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
-  const int splits = (int)std::min<int64_t>(I, 64);
-  const int64_t i_per = (I + splits - 1) / splits;
   const int64_t n = J * rank;
-  dim3 grid((unsigned)ceil_div(n, 256), (unsigned)splits);
-  rankdiag_axis0_kernel<<<grid, 256, 0, st>>>(Y, I, J, rank, F, i_per, (float*)ws);
-  TLA_LAUNCH_CHECK();
-  sum_splits_kernel<<<ceil_div(n, 256), 256, 0, st>>>((const float*)ws, n, splits, out);
+  rankdiag_axis0_kernel<<<(unsigned)ceil_div(n, 64), 256, 0, st>>>(Y, I, J, rank, F, out);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }
