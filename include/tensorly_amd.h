@@ -124,8 +124,10 @@ int tla_gram64(int dtype, const void* A, int64_t rows, int rank, double* G, void
 /* ---------------------------------------------------------------------
  * ALS factor update (tensorly/decomposition/_cp.py:411-427):
  *   V = (prod_{i != mode} G_i + l2_reg I) * (w w^T)          (FP64)
- *   factor = mttkrp @ inv(V)          (FP64 Cholesky; LU with partial
- *                                      pivoting if V is not positive definite)
+ *   factor = mttkrp @ inv(V)          (FP64 SPD elimination, i.e. the
+ *                                      inverse from the LDL^T / Cholesky
+ *                                      pivots; LU with partial pivoting if
+ *                                      V is not positive definite)
  * and, fused, the FP64 Gram of the new factor (gram_out) and, if
  * iprod_out != NULL, <mttkrp, factor> (the error_calc inner product,
  * _cp.py:222).  grams: host array of ndim device pointers to R x R FP64.
@@ -250,6 +252,45 @@ int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const float* mlp_par
                                 float* features, void* workspace, size_t workspace_bytes, void* stream);
 int tla_extrap_features(const double* errors, const float* alphas, const int* accepted, int s, float* out,
                         void* stream);
+
+/* ---------------------------------------------------------------------
+ * Batched CP-ALS: n_problems independent dense 3-way float32 problems
+ * (every mode size 1..64, rank 1..16) run to convergence in ONE persistent
+ * launch; each workgroup pulls problems from a device queue.  Per problem
+ * it is the loop of tensorly/decomposition/_cp.py:394-531 (weights fixed at
+ * ones, no normalisation, tol on |Δ rec_error| as cvg_criterion
+ * 'abs_rec_error', :506-521) with
+ *   policy 0  plain ALS;
+ *          1  the reference line search (_cp.py:333-336,401-403,448-483);
+ *          2  learned (mlp_params) or fixed (alpha_fixed, mlp_params NULL)
+ *             extrapolation, verified by the next sweep, after `warmup`
+ *             plain sweeps;
+ *          3  probing: as 2, but at each of the first max_decisions
+ *             decisions every step in probe_alphas[n_probe] is tried (one
+ *             sweep each, errors to probe_err[p][d][g], features to
+ *             features[p][d][16]) and the best one is taken.
+ * Layout (all device): X packed, tensor p at X + x_off[p] (C-order,
+ * shapes[3p..3p+2]); factors packed, problem p's A | B | C (row-major
+ * I_m x ranks[p]) at factors + f_off[p], updated in place; f_prev / f_save
+ * are scratch of the same size; order[] is the queue order (a permutation);
+ * queue is one device int (reset by the call).  History arrays (nullable
+ * for policies 0 and 1) are [n_problems][max_sweeps].  Outputs per problem:
+ * sweeps, tensor passes, final relative error, extrapolations tried and
+ * kept, status (0 ok, 2 singular, 3 non-finite), ticks (nullable, 100 MHz
+ * wall clock).  waves: 2, 4 or 8 per workgroup; grid: workgroups.
+ * Needs no workspace beyond the arrays above and allocates nothing, so the
+ * call can be captured in a HIP graph (the queue reset is a kernel).
+ * tla_batched_cp_max_blocks_per_cu: resident workgroups per CU for `waves`.
+ * ------------------------------------------------------------------- */
+int tla_batched_cp_max_blocks_per_cu(int waves);
+int tla_batched_cp_als(int n_problems, const float* X, const int64_t* x_off, const int* shapes, const int* ranks,
+                       float* factors, const int64_t* f_off, float* f_prev, float* f_save, const int* order,
+                       int* queue, int policy, int max_sweeps, double tol, int warmup, const float* mlp_params,
+                       float alpha_fixed, float alpha_max, const float* probe_alphas, int n_probe,
+                       int max_decisions, double* hist_err, float* hist_alpha, int* hist_acc, float* features,
+                       double* probe_err, int* sweeps_out, int* passes_out, double* err_out, int* n_extrap_out,
+                       int* n_accept_out, int* status_out, long long* ticks_out, int waves, int grid,
+                       void* stream);
 
 #ifdef __cplusplus
 }

@@ -531,6 +531,8 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
                      for i in range(tensor.ndim)]
             new_err = error_calc(tensor, norm_tensor, new_w, new_f)
             rec["jump"] = jump
+            rec["new_error"] = new_err / norm_tensor
+            rec["last_error"] = rec_errors[-1]
             if new_err / norm_tensor < rec_errors[-1]:
                 factors, weights = new_f, new_w
                 unnorm = new_err

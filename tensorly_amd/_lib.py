@@ -68,6 +68,10 @@ _SIGNATURES = {
     "tla_cp_als_extrap_workspace_size": (c_sz, [c_p]),
     "tla_cp_als_run_extrapolated": (c_i, [c_p, c_i, c_p, c_f, c_f, c_i, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "tla_extrap_features": (c_i, [c_p, c_p, c_p, c_i, c_p, c_p]),
+    "tla_batched_cp_max_blocks_per_cu": (c_i, [c_i]),
+    "tla_batched_cp_als": (c_i, [c_i, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i, c_d, c_i, c_p, c_f,
+                                 c_f, c_p, c_i, c_i, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i,
+                                 c_i, c_p]),
 }
 
 

@@ -1,5 +1,6 @@
 // FP64 pieces of the ALS sweep: factor Grams, the Hadamard-of-Grams normal
-// equations with a Cholesky solve (LU fallback), the squared norm of the
+// equations solved by symmetric positive definite elimination (LU with
+// partial pivoting as the fallback), the squared norm of the
 // tensor and the reconstruction error.  Everything here is R x R or I x R
 // sized -- launch/latency bound -- so each step is one small fused kernel and
 // the R x R work is done redundantly per workgroup rather than paying a
@@ -10,7 +11,6 @@
 
 namespace tla {
 
-constexpr int kMaxRank = 64;
 constexpr int kSolveRows = 64;  // factor rows handled by one solve workgroup
 
 template <typename T>
@@ -87,8 +87,8 @@ __global__ __launch_bounds__(256) void gram_finalize_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Normal-equations solve.  Every block rebuilds V, factors it and forms
-// V^-1 in LDS (R <= 64: a few microseconds), then handles kSolveRows rows of
+// Normal-equations solve.  Every block rebuilds V and inverts it in
+// registers (R <= 64: a few microseconds), then handles kSolveRows rows of
 // the new factor, its partial Gram and partial <M, A>.
 // ---------------------------------------------------------------------------
 struct SolveArgs {
@@ -107,51 +107,95 @@ struct SolveArgs {
   double* iprod_out;
 };
 
-// In-place Cholesky of the R x R matrix in LDS (lower triangle).  Returns
-// false if a pivot is not positive and finite.
-__device__ bool lds_cholesky(double* V, int R, int* flag) {
-  const int tid = threadIdx.x;
-  if (tid == 0) *flag = 0;
-  __syncthreads();
-  for (int k = 0; k < R; ++k) {
-    if (tid == 0) {
-      const double d = V[k * R + k];
-      if (!(d > 0.0) || !isfinite(d)) *flag = 1;
-      V[k * R + k] = sqrt(fmax(d, 0.0));
-    }
-    __syncthreads();
-    if (*flag) return false;
-    const double dk = V[k * R + k];
-    for (int i = k + 1 + tid; i < R; i += blockDim.x) V[i * R + k] /= dk;
-    __syncthreads();
-    const int m = R - k - 1;
-    for (int e = tid; e < m * m; e += blockDim.x) {
-      const int i = k + 1 + e / m, j = k + 1 + e % m;
-      if (j <= i) V[i * R + j] -= V[i * R + k] * V[j * R + k];
-    }
-    __syncthreads();
-  }
-  return true;
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Inverse of L L^T into Vi (full matrix): thread s solves L L^T x = e_s.
-__device__ void lds_chol_inverse(const double* L, double* Vi, int R) {
-  const int s = threadIdx.x;
-  if (s < R) {
-    // forward: L y = e_s, stored in column s of Vi
-    for (int i = 0; i < R; ++i) {
-      double v = (i == s) ? 1.0 : 0.0;
-      for (int j = 0; j < i; ++j) v -= L[i * R + j] * Vi[j * R + s];
-      Vi[i * R + s] = v / L[i * R + i];
+template <typename T>
+__device__ __forceinline__ double v_entry(const SolveArgs& a, int r, int s) {
+  double v = 1.0;
+  for (int m = 0; m < a.ndim; ++m)
+    if (m != a.mode) v *= a.grams[m][r * a.R + s];
+  if (r == s) v += a.l2;
+  if (a.weights) v *= ld<T>(a.weights, r) * ld<T>(a.weights, s);
+  return v;
+}
+
+// V^-1 of the symmetric positive definite V by Gauss-Jordan without pivoting,
+// held in registers: wave w owns rows w, w + 4, ... of [V | I], lane l owns
+// columns l and l + 64.  Per elimination step the owner of the pivot row
+// publishes it through LDS (double-buffered, one barrier per step) and every
+// row reads its multiplier from the pivot column with a lane read.  For SPD V
+// the pivots are the (positive) Schur complement diagonals; a pivot that is
+// not positive and finite returns false (the caller falls back to partial
+// pivoting).  Needs blockDim.x == 256 and R <= 64.
+template <typename T>
+__device__ bool spd_gj_inverse(const SolveArgs& a, double* Vi, double* pbuf, int* flag) {
+  const int R = a.R, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nq = (R - w + 3) >> 2;
+  double x[16][2];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = w + 4 * q;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = lane + 64 * h;
+      double v = 0.0;
+      if (q < nq) v = (c < R) ? v_entry<T>(a, i, c) : ((c - R == i) ? 1.0 : 0.0);
+      x[q][h] = v;
     }
-    // backward: L^T x = y
-    for (int i = R - 1; i >= 0; --i) {
-      double v = Vi[i * R + s];
-      for (int j = i + 1; j < R; ++j) v -= L[j * R + i] * Vi[j * R + s];
-      Vi[i * R + s] = v / L[i * R + i];
+  }
+  for (int k = 0; k < R; ++k) {
+    const int qk = k >> 2, kl = k & 63, kh = k >> 6;
+    double* buf = pbuf + (k & 1) * 128;
+    if (w == (k & 3)) {
+      double r0 = 0.0, r1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q == qk) {
+          r0 = x[q][0];
+          r1 = x[q][1];
+        }
+      const double piv = readlane_f64(kh ? r1 : r0, kl);
+      const bool bad = !(piv > 0.0) || !isfinite(piv);
+      r0 /= piv;
+      r1 /= piv;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q == qk) {
+          x[q][0] = r0;
+          x[q][1] = r1;
+        }
+      buf[lane] = r0;
+      buf[64 + lane] = r1;
+      if (lane == 0) flag[k & 1] = bad ? 1 : 0;
+    }
+    __syncthreads();
+    if (flag[k & 1]) return false;
+    const double p0 = buf[lane], p1 = buf[64 + lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (q < nq && w + 4 * q != k) {
+        const double f = readlane_f64(kh ? x[q][1] : x[q][0], kl);
+        x[q][0] = fma(-f, p0, x[q][0]);
+        x[q][1] = fma(-f, p1, x[q][1]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = w + 4 * q;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = lane + 64 * h;
+      if (q < nq && c >= R && c < 2 * R) Vi[i * R + (c - R)] = x[q][h];
     }
   }
   __syncthreads();
+  return true;
 }
 
 // Gauss-Jordan inverse with partial pivoting (fallback for V that is not
@@ -222,19 +266,16 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __shared__ double Lbuf[kMaxRank * kMaxRank];
   __shared__ double Vi[kMaxRank * kMaxRank];
   __shared__ double Mr[kSolveRows * kMaxRank];
-  __shared__ int flag, piv, ticket;
+  __shared__ int flag, piv, ticket, pflag[2];
   const int R = a.R, tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
   const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
   // this block's MTTKRP rows, staged once (coalesced)
   for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
-  build_v<T>(a, Lbuf);
   int status = 0;
-  if (!lds_cholesky(Lbuf, R, &flag)) {
+  if (!spd_gj_inverse<T>(a, Vi, Lbuf, pflag)) {
     build_v<T>(a, Lbuf);
     status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
-  } else {
-    lds_chol_inverse(Lbuf, Vi, R);
   }
   if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
   __syncthreads();

@@ -40,6 +40,7 @@ void set_error(const char* fmt, ...);
   } while (0)
 
 constexpr int kMaxOrder = 8;
+constexpr int kMaxRank = 64;  // CP rank limit of the ALS solve / Gram kernels
 constexpr int kWave = 64;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));

@@ -4,7 +4,7 @@
 //
 // Per sweep and per free mode n:
 //   M = MTTKRP(X, factors, weights, n)          (fused kernel, one tensor pass)
-//   factors[n] = M V^-1, V = (*_{m != n} G_m + l2 I) * w w^T   (FP64 Cholesky)
+//   factors[n] = M V^-1, V = (*_{m != n} G_m + l2 I) * w w^T   (FP64 SPD solve)
 //   G_n = factors[n]^T factors[n]                (FP64, fused into the solve)
 // and after the last mode (whose MTTKRP is kept) the error of
 // _cp.py:217-225 from the Grams: sqrt(|‖X‖² + ‖M‖² − 2<X, M>|) / ‖X‖.
@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "common.h"
+#include "extrap.h"
 
 namespace tla {
 
@@ -367,38 +368,10 @@ extern "C" int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t 
 // ---------------------------------------------------------------------------
 namespace tla {
 
-constexpr int kMlpIn = 16, kMlpH = 64;
-constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;
-
 struct FactorPtrs {
   float* p[kMaxOrder];
   int64_t n[kMaxOrder];  // elements
 };
-
-__device__ void extrap_features(const double* errors, const float* alphas, const int* accepted, int s, float* f) {
-  double e[5];
-  for (int k = 0; k < 5; ++k) e[k] = errors[max(s - k, 0)];
-  double d[4];
-  for (int k = 0; k < 4; ++k) d[k] = e[k + 1] - e[k];
-  const double e0 = fmax(e[0], 1e-30);
-  f[0] = (float)log10(e0);
-  for (int k = 0; k < 4; ++k) f[1 + k] = (float)log10(fmax(fabs(d[k]), 1e-30) / e0);
-  for (int k = 0; k < 3; ++k) f[5 + k] = (d[k + 1] != 0.0) ? (float)fmin(fmax(d[k] / d[k + 1], -2.0), 2.0) : 0.0f;
-  const float a1 = (s >= 1) ? alphas[s - 1] : 0.0f;
-  const float a2 = (s >= 3) ? alphas[s - 3] : 0.0f;
-  f[8] = tanhf(a1 / 4.0f);
-  f[9] = tanhf(a2 / 4.0f);
-  f[10] = (s >= 1) ? (float)accepted[s - 1] : 0.5f;
-  float acc = 0.0f;
-  int cnt = 0;
-  for (int k = 1; k <= 7; k += 2)
-    if (s - k >= 0) { acc += (float)accepted[s - k]; ++cnt; }
-  f[11] = cnt ? acc / cnt : 0.5f;
-  f[12] = (float)(log10((double)s + 1.0) / 3.0);
-  f[13] = (d[0] > 0.0) ? 1.0f : ((d[0] < 0.0) ? -1.0f : 0.0f);
-  f[14] = (float)(fmin(fmax((e[4] - e[0]) / (4.0 * fmax(fabs(d[0]), 1e-30)), -4.0), 4.0) / 4.0);
-  f[15] = 1.0f;
-}
 
 // One block: features of sweep s, MLP forward, alpha -> alphas[s + 1].
 __global__ void extrap_alpha_kernel(const double* errors, float* alphas, const int* accepted, int s,
@@ -416,43 +389,77 @@ __global__ void extrap_alpha_kernel(const double* errors, float* alphas, const i
     if (t == 0) alphas[s + 1] = alpha_fixed;
     return;
   }
-  const float* W1 = params;
-  const float* b1 = W1 + kMlpH * kMlpIn;
-  const float* W2 = b1 + kMlpH;
-  const float* b2 = W2 + kMlpH * kMlpH;
-  const float* w3 = b2 + kMlpH;
-  const float* b3 = w3 + kMlpH;
-  if (t < kMlpH) {
-    float z = b1[t];
-    for (int i = 0; i < kMlpIn; ++i) z = fmaf(W1[t * kMlpIn + i], f[i], z);
-    h1[t] = tanhf(z);
-  }
-  __syncthreads();
-  if (t < kMlpH) {
-    float z = b2[t];
-    for (int j = 0; j < kMlpH; ++j) z = fmaf(W2[t * kMlpH + j], h1[j], z);
-    part[t] = w3[t] * tanhf(z);
-  }
-  __syncthreads();
-  if (t == 0) {
-    float y = b3[0];
-    for (int k = 0; k < kMlpH; ++k) y += part[k];
-    float a = exp2f(y) - 1.0f;
-    a = fminf(fmaxf(a, 0.0f), alpha_max);
-    if (!(a == a)) a = 0.0f;
-    alphas[s + 1] = a;
-  }
+  const float a = mlp_alpha_block(params, f, h1, part, alpha_max);
+  if (t == 0) alphas[s + 1] = a;
 }
 
-// A_save = A; A = A + alpha (A - A_prev)   (blockIdx.y = factor)
-__global__ void extrapolate_kernel(FactorPtrs A, FactorPtrs Aprev, FactorPtrs Asave, const float* alphas, int slot) {
-  const int m = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= A.n[m]) return;
-  const float a = alphas[slot];
-  const float x = A.p[m][e];
-  Asave.p[m][e] = x;
-  A.p[m][e] = x + a * (x - Aprev.p[m][e]);
+constexpr int kExtrapRows = 64;
+
+struct ExtrapArgs {
+  FactorPtrs A, Aprev, Asave;
+  double* grams[kMaxOrder];
+  double* gsave;  // ndim x R x R
+  double* gpart;  // ndim x maxblk x R x R
+  int* counters;  // ndim, zero on entry, left at zero
+  const float* alphas;
+  int slot, R, ndim, maxblk;
+  int64_t rows[kMaxOrder];
+};
+
+// A_save = A; A = A + alpha (A - A_prev) for kExtrapRows rows of factor
+// blockIdx.y, with the FP64 Gram of the new factor built on the way (per-block
+// partials, the last block of each factor sums them in block order).  Block
+// (0, 0) first saves the current Grams so that a rejected step can restore
+// them without recomputing.
+__global__ __launch_bounds__(256) void extrapolate_gram_kernel(ExtrapArgs a) {
+  __shared__ double tile[kExtrapRows * kMaxRank];
+  __shared__ int ticket;
+  const int m = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x, R = a.R;
+  const int64_t rows = a.rows[m];
+  const int nblk = (int)((rows + kExtrapRows - 1) / kExtrapRows);
+  if (blk >= nblk) return;
+  if (m == 0 && blk == 0)
+    for (int e = tid; e < a.ndim * R * R; e += blockDim.x) a.gsave[e] = a.grams[e / (R * R)][e % (R * R)];
+  const float alpha = a.alphas[a.slot];
+  const int64_t r0 = (int64_t)blk * kExtrapRows;
+  const int nr = (int)min((int64_t)kExtrapRows, rows - r0);
+  float* A = a.A.p[m] + r0 * R;
+  const float* P = a.Aprev.p[m] + r0 * R;
+  float* S = a.Asave.p[m] + r0 * R;
+  for (int e = tid; e < nr * R; e += blockDim.x) {
+    const float x = A[e];
+    const float y = x + alpha * (x - P[e]);
+    S[e] = x;
+    A[e] = y;
+    tile[e] = (double)y;
+  }
+  __syncthreads();
+  double* part = a.gpart + ((int64_t)m * a.maxblk + blk) * R * R;
+  for (int e = tid; e < R * R; e += blockDim.x) {
+    const int r = e / R, q = e % R;
+    double s0 = 0.0, s1 = 0.0;
+    int i = 0;
+    for (; i + 1 < nr; i += 2) {
+      s0 = fma(tile[i * R + r], tile[i * R + q], s0);
+      s1 = fma(tile[(i + 1) * R + r], tile[(i + 1) * R + q], s1);
+    }
+    if (i < nr) s0 = fma(tile[i * R + r], tile[i * R + q], s0);
+    part[e] = s0 + s1;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) ticket = atomicAdd(a.counters + m, 1);
+  __syncthreads();
+  if (ticket != nblk - 1) return;
+  __threadfence();
+  const double* base = a.gpart + (int64_t)m * a.maxblk * R * R;
+  for (int e = tid; e < R * R; e += blockDim.x) {
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b)
+      acc += __hip_atomic_load(base + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a.grams[m][e] = acc;
+  }
+  if (tid == 0) a.counters[m] = 0;
 }
 
 __global__ void copy_factors_kernel(FactorPtrs dst, FactorPtrs src) {
@@ -462,15 +469,25 @@ __global__ void copy_factors_kernel(FactorPtrs dst, FactorPtrs src) {
   dst.p[m][e] = src.p[m][e];
 }
 
-// keep the extrapolated step iff errors[s + 1] < errors[s]; else restore
-__global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, double* errors, int* accepted, int s) {
+// Keep the extrapolated step iff errors[s + 1] < errors[s]; else restore the
+// saved factors and Grams and carry the previous error forward.
+struct GramPtrArray {
+  double* p[kMaxOrder];
+};
+
+__global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, GramPtrArray grams, const double* gsave,
+                                        int ndim, int R, double* errors, int* accepted, int s) {
   const bool ok = errors[s + 1] < errors[s];
   const int m = blockIdx.y;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (!ok && e < A.n[m]) A.p[m][e] = Asave.p[m][e];
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-    accepted[s + 1] = ok ? 1 : 0;
-    if (!ok) errors[s + 1] = errors[s];
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (!ok)
+      for (int i = threadIdx.x; i < ndim * R * R; i += blockDim.x) grams.p[i / (R * R)][i % (R * R)] = gsave[i];
+    if (threadIdx.x == 0) {
+      accepted[s + 1] = ok ? 1 : 0;
+      if (!ok) errors[s + 1] = errors[s];
+    }
   }
 }
 
@@ -482,8 +499,13 @@ extern "C" size_t tla_cp_als_extrap_workspace_size(void* handle) {
   CpAls* h = (CpAls*)handle;
   if (!h) return 0;
   size_t tot = 0;
-  for (int m = 0; m < h->ndim; ++m) tot += align256(size_t(h->shape[m]) * h->R * 4);
-  return 2 * tot;
+  int64_t maxblk = 0;
+  for (int m = 0; m < h->ndim; ++m) {
+    tot += align256(size_t(h->shape[m]) * h->R * 4);
+    maxblk = std::max(maxblk, (h->shape[m] + kExtrapRows - 1) / kExtrapRows);
+  }
+  const size_t rr = size_t(h->R) * h->R * 8;
+  return 2 * tot + align256(h->ndim * rr) + align256(h->ndim * maxblk * rr) + align256(size_t(h->ndim) * 4);
 }
 
 extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const float* mlp_params, float alpha_fixed,
@@ -514,18 +536,36 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
     Asave.p[m] = (float*)(base + off);
     off += align256(size_t(A.n[m]) * 4);
   }
+  const int R = h->R;
+  int64_t maxblk = 0;
+  for (int m = 0; m < h->ndim; ++m) maxblk = std::max(maxblk, (h->shape[m] + kExtrapRows - 1) / kExtrapRows);
+  ExtrapArgs ea{};
+  ea.A = A;
+  ea.Aprev = Aprev;
+  ea.Asave = Asave;
+  GramPtrArray gpa{};
+  for (int m = 0; m < h->ndim; ++m) {
+    ea.grams[m] = h->grams[m];
+    gpa.p[m] = h->grams[m];
+    ea.rows[m] = h->shape[m];
+  }
+  ea.gsave = (double*)(base + off);
+  off += align256(size_t(h->ndim) * R * R * 8);
+  ea.gpart = (double*)(base + off);
+  off += align256(size_t(h->ndim) * maxblk * R * R * 8);
+  ea.counters = (int*)(base + off);
+  ea.alphas = alphas;
+  ea.R = R;
+  ea.ndim = h->ndim;
+  ea.maxblk = (int)maxblk;
+  zero_ints_kernel<<<1, 64, 0, st>>>(ea.counters, h->ndim);
+  TLA_LAUNCH_CHECK();
   const dim3 fgrid((unsigned)ceil_div(maxn, 256), (unsigned)h->ndim);
+  const dim3 egrid((unsigned)maxblk, (unsigned)h->ndim);
   auto sweep = [&](int slot) -> int {
     int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
     if (rc) return rc;
     return tla_cp_error(gp, h->ndim, h->weights, TLA_F32, h->R, h->normx2, h->iprod, errors, slot, st);
-  };
-  auto refresh = [&]() -> int {
-    for (int m = 0; m < h->ndim; ++m) {
-      int rc = tla_gram64(TLA_F32, h->factors[m], h->shape[m], h->R, h->grams[m], h->ws, h->ws_bytes, st);
-      if (rc) return rc;
-    }
-    return TLA_OK;
   };
   int s = 0;
   // plain warm-up sweeps (flags/alphas for these slots are set by the caller)
@@ -533,6 +573,9 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
     int rc = sweep(s);
     if (rc) return rc;
   }
+  // Per pair: save A_prev, plain sweep, step from the network, extrapolate
+  // (+ Grams), sweep from there, keep or restore.  The Grams stay current
+  // throughout, so no separate Gram pass is needed.
   while (s < n_sweeps) {
     copy_factors_kernel<<<fgrid, 256, 0, st>>>(Aprev, A);
     TLA_LAUNCH_CHECK();
@@ -542,13 +585,12 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
     extrap_alpha_kernel<<<1, 64, 0, st>>>(errors, alphas, accepted, s, mlp_params, alpha_fixed, alpha_max,
                                            features, 1);
     TLA_LAUNCH_CHECK();
-    extrapolate_kernel<<<fgrid, 256, 0, st>>>(A, Aprev, Asave, alphas, s + 1);
+    ea.slot = s + 1;
+    extrapolate_gram_kernel<<<egrid, 256, 0, st>>>(ea);
     TLA_LAUNCH_CHECK();
-    if ((rc = refresh())) return rc;
     if ((rc = sweep(s + 1))) return rc;
-    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, errors, accepted, s);
+    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, gpa, ea.gsave, h->ndim, R, errors, accepted, s);
     TLA_LAUNCH_CHECK();
-    if ((rc = refresh())) return rc;
     s += 2;
   }
   return TLA_OK;

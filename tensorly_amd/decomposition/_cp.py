@@ -3,7 +3,7 @@ kernels.
 
 :func:`parafac` keeps the reference signature (``_cp.py:230-250``) and return
 convention (``:532-543``); the mode loop runs in the native driver
-(``tla_cp_als_*``): fused MTTKRP, FP64 Hadamard-of-Grams + Cholesky solve,
+(``tla_cp_als_*``): fused MTTKRP, FP64 Hadamard-of-Grams + SPD solve,
 FP64 fit from the last mode's MTTKRP (``:217-225``), with no host
 synchronisation unless ``tol``, ``callback``, ``verbose`` or the line search
 ask for the error on the host (as the reference does).
