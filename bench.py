@@ -10,7 +10,15 @@ output) per sweep / wall time.
 Also reported on the same JSON line:
   * time_to_fit: wall time until the relative error is within 1e-4 of the FP64
     oracle's error after 100 sweeps (tests/golden/cp512_oracle.npz), for plain
-    FP32 sweeps and for the fast path, and their ratio;
+    FP32 sweeps, plain bf16x3 dimension-tree sweeps and bf16x3 dimension-tree
+    sweeps with the learned extrapolation (trained network in
+    tensorly_amd/models/extrapolation_mlp.npz); speedup_vs_fp32 is
+    fp32_plain / bf16x3+learned;
+  * batched: the 4096-problem batched workload (sizes U{8..64}, ranks
+    U{2..16}, congruence 0.9, 1 % noise, seed 4) run to |delta rel. error| <
+    1e-7 in one persistent launch per policy (plain ALS, the reference line
+    search, learned extrapolation): wall time, sweeps, tensor passes and the
+    fallback (rejected-step) rate;
   * roofline: the dominant kernel's algorithmic bytes / average duration
     (HIP events on the launch stream) against 8 TB/s;
   * cpu_baseline: the numpy restatement of the reference path (unfold copy,
@@ -46,6 +54,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--quick", action="store_true", help="skip time-to-fit and the CPU baseline")
     p.add_argument("--fit-sweeps", type=int, default=100)
+    p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     return p.parse_args()
 
 
@@ -87,6 +96,97 @@ def time_to_fit(engine, n_sweeps, target_err, torch):
     return evs[0].elapsed_time(evs[s + 1]) / 1e3, s + 1, e
 
 
+def load_model(torch, dev):
+    from tensorly_amd.learned import ExtrapolationMLP, default_model_path
+    path = default_model_path()
+    if not os.path.exists(path):
+        return None
+    return ExtrapolationMLP(np.load(path)["params"], device=dev)
+
+
+def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch):
+    """bf16x3 dimension-tree sweeps with the learned step: find the first
+    sweep slot whose error reaches the target, then time a fresh run of
+    exactly that many slots (the run is deterministic)."""
+    from tensorly_amd.decomposition._cp import CpAlsEngine
+    from tensorly_amd.learned import LearnedExtrapolation, run_extrapolated_sweeps
+    pol = LearnedExtrapolation(net, alpha_max=16.0, warmup=3)
+    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    errs, acc, alphas, _ = run_extrapolated_sweeps(eng, n_slots, pol)
+    e = errs.cpu().numpy()
+    a = acc.cpu().numpy()
+    al = alphas.cpu().numpy()
+    eng.close()
+    hit = np.nonzero(e <= target_err)[0]
+    out = {"final_error": float(e[-1]), "slots_run": int(n_slots)}
+    if len(hit) == 0:
+        out.update({"seconds": None, "sweeps": None})
+        return out
+    k = int(hit[0]) + 1
+    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    errs2, _, _, _ = run_extrapolated_sweeps(eng, k, pol)
+    e1.record()
+    torch.cuda.synchronize()
+    eng.close()
+    ext = a[3:k]
+    steps = al[3:k][1::2] if k > 4 else al[:0]
+    out.update({"seconds": e0.elapsed_time(e1) / 1e3, "sweeps": k,
+                "error_at_hit": float(errs2.cpu().numpy()[k - 1]),
+                "extrapolations": int(len(ext[1::2])), "rejected": int((ext[1::2] == 0).sum()),
+                "mean_alpha": float(np.mean(steps)) if len(steps) else None})
+    return out
+
+
+def batched_report(torch, dev, net, log):
+    """The 4096-problem batched workload, one launch per policy."""
+    from tensorly_amd import synthetic
+    from tensorly_amd.batched import BatchedCP
+    t0 = time.time()
+    probs = synthetic.batched_cp(n_problems=4096, seed=4)
+    b = BatchedCP([x for x, _ in probs], [r for _, r in probs], device=dev)
+    log(f"[bench] batched: {len(probs)} problems, {b.X.numel() * 4 / 1e9:.3f} GB packed "
+        f"in {time.time() - t0:.1f}s")
+    table = b.autotune(sweeps=5, repeats=2)
+    f0 = b.random_init(seed=0)
+    max_sweeps = 1000
+    res = {"problems": len(probs), "tensor_bytes": int(b.X.numel() * 4), "tol": 1e-7, "max_sweeps": max_sweeps,
+           "autotune": {"chosen": b.config, "table": table}}
+    policies = [("plain", {}), ("linesearch", {})]
+    if net is not None:
+        policies.append(("learned", {"model": net, "alpha_max": 16.0}))
+    for name, kw in policies:
+        out = b.make_outputs(max_sweeps, history=(name == "learned"))
+        f = f0.clone()
+        b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)  # warm-up
+        f.copy_(f0)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)
+        e1.record()
+        torch.cuda.synchronize()
+        sw = out["sweeps"].cpu().numpy()
+        ps = out["passes"].cpu().numpy()
+        ne = int(out["n_extrap"].sum())
+        na = int(out["n_accept"].sum())
+        nbytes = float(np.sum(ps.astype(np.float64) * np.prod(b.shapes, axis=1) * 4))
+        ms = e0.elapsed_time(e1)
+        res[name] = {"ms": round(ms, 2), "sweeps_mean": round(float(sw.mean()), 2),
+                     "sweeps_median": float(np.median(sw)), "sweeps_max": int(sw.max()),
+                     "not_converged": int((sw >= max_sweeps).sum()), "tensor_passes": int(ps.sum()),
+                     "tensor_GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "failed": int((out["status"].cpu().numpy() != 0).sum()),
+                     "mean_final_error": float(out["error"].cpu().numpy().mean()),
+                     "extrapolations": ne, "fallback_rate": (1.0 - na / ne) if ne else None}
+        log(f"[bench] batched {name}: {res[name]}")
+    return res
+
+
 def cpu_baseline(x32):
     """One float32 ALS sweep of the numpy restatement, median of 3 after a
     warm-up."""
@@ -123,8 +223,8 @@ def cpu_baseline(x32):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only "
-              "(see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only"
+              , file=sys.stderr)
         sys.exit(2)
     import torch
     import tensorly_amd as tla
@@ -192,12 +292,23 @@ def main():
                 res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
                 eng.close()
                 log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
-            speed = None
-            if res["fp32_plain"]["seconds"] and res["bf16x3_dimtree"]["seconds"]:
-                speed = res["fp32_plain"]["seconds"] / res["bf16x3_dimtree"]["seconds"]
+            net = load_model(torch, dev)
+            if net is not None:
+                res["bf16x3_dimtree_learned"] = learned_time_to_fit(X, ones, init, net, args.fit_sweeps, target, torch)
+                log(f"[bench] time-to-fit bf16x3_dimtree_learned: {res['bf16x3_dimtree_learned']}")
+            else:
+                res["bf16x3_dimtree_learned"] = {"seconds": None, "note": "no trained model file"}
+            best = res["bf16x3_dimtree_learned"]["seconds"]
+            base = res["fp32_plain"]["seconds"]
             ttf = {"target_error": target, "oracle_error_100_sweeps": float(ref[-1]), **res,
-                   "speedup_vs_fp32": speed}
+                   "speedup_vs_fp32": (base / best) if (base and best) else None,
+                   "speedup_plain_bf16x3_vs_fp32": (base / res["bf16x3_dimtree"]["seconds"])
+                   if (base and res["bf16x3_dimtree"]["seconds"]) else None}
     fast.close()
+
+    batched = None
+    if not (args.quick or args.no_batched):
+        batched = batched_report(torch, dev, load_model(torch, dev), log)
 
     cpu = None
     if not (args.no_cpu or args.quick):
@@ -225,6 +336,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "kernels": kernels,
         "time_to_fit": ttf,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -265,10 +265,12 @@ int tla_extrap_features(const double* errors, const float* alphas, const int* ac
  *          2  learned (mlp_params) or fixed (alpha_fixed, mlp_params NULL)
  *             extrapolation, verified by the next sweep, after `warmup`
  *             plain sweeps;
- *          3  probing: as 2, but at each of the first max_decisions
- *             decisions every step in probe_alphas[n_probe] is tried (one
- *             sweep each, errors to probe_err[p][d][g], features to
- *             features[p][d][16]) and the best one is taken.
+ *          3  probing: as 2, but at decisions first_decision ..
+ *             first_decision + max_decisions - 1 every step in
+ *             probe_alphas[n_probe] is tried (one sweep each, errors to
+ *             probe_err[p][d][g], features to features[p][d][16]) and the
+ *             best one is taken; other decisions use the network (if
+ *             mlp_params) or alpha_fixed.
  * Layout (all device): X packed, tensor p at X + x_off[p] (C-order,
  * shapes[3p..3p+2]); factors packed, problem p's A | B | C (row-major
  * I_m x ranks[p]) at factors + f_off[p], updated in place; f_prev / f_save
@@ -287,7 +289,7 @@ int tla_batched_cp_als(int n_problems, const float* X, const int64_t* x_off, con
                        float* factors, const int64_t* f_off, float* f_prev, float* f_save, const int* order,
                        int* queue, int policy, int max_sweeps, double tol, int warmup, const float* mlp_params,
                        float alpha_fixed, float alpha_max, const float* probe_alphas, int n_probe,
-                       int max_decisions, double* hist_err, float* hist_alpha, int* hist_acc, float* features,
+                       int max_decisions, int first_decision, double* hist_err, float* hist_alpha, int* hist_acc, float* features,
                        double* probe_err, int* sweeps_out, int* passes_out, double* err_out, int* n_extrap_out,
                        int* n_accept_out, int* status_out, long long* ticks_out, int waves, int grid,
                        void* stream);

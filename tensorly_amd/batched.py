@@ -165,7 +165,8 @@ class BatchedCP:
         return out
 
     def launch(self, factors, out, policy="plain", max_sweeps=1000, tol=1e-7, warmup=3, model=None,
-               alpha_fixed=0.0, alpha_max=8.0, probe_alphas=None, max_decisions=0, waves=None, grid=None):
+               alpha_fixed=0.0, alpha_max=16.0, probe_alphas=None, max_decisions=0, first_decision=0, waves=None,
+               grid=None):
         """Enqueue one batched run on the current stream (no host sync).
         ``factors`` (packed, float32) is updated in place; ``out`` comes from
         :meth:`make_outputs`.  Safe to capture in a graph."""
@@ -186,7 +187,7 @@ class BatchedCP:
             self.n_problems, ptr(self.X), ptr(self.x_off), ptr(self.shape_d), ptr(self.rank_d), ptr(factors),
             ptr(self.f_off), ptr(self.f_prev), ptr(self.f_save), ptr(self.order), ptr(self.queue), pol,
             int(max_sweeps), float(tol), int(warmup), ptr(params), float(alpha_fixed), float(alpha_max),
-            ptr(probe_alphas), n_probe, int(max_decisions), ptr(out.get("hist_err")), ptr(out.get("hist_alpha")),
+            ptr(probe_alphas), n_probe, int(max_decisions), int(first_decision), ptr(out.get("hist_err")), ptr(out.get("hist_alpha")),
             ptr(out.get("hist_acc")), ptr(out.get("features")), ptr(out.get("probe_err")), ptr(out["sweeps"]),
             ptr(out["passes"]), ptr(out["error"]), ptr(out["n_extrap"]), ptr(out["n_accept"]), ptr(out["status"]),
             ptr(out["ticks"]), int(waves), int(grid), _lib.stream(self.device)), "batched_cp_als")

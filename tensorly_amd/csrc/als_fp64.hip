@@ -105,6 +105,11 @@ struct SolveArgs {
   int* counter;          // nullable: zero-initialised block counter -> the last block finalises
   double* gram_out;
   double* iprod_out;
+  // optional, last mode only: errors[err_slot] = relative error of the model
+  // after this update (needs iprod_out), computed by the finalising block
+  const double* normx2;
+  double* errors;
+  int err_slot;
 };
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -114,17 +119,9 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <typename T>
-__device__ __forceinline__ double v_entry(const SolveArgs& a, int r, int s) {
-  double v = 1.0;
-  for (int m = 0; m < a.ndim; ++m)
-    if (m != a.mode) v *= a.grams[m][r * a.R + s];
-  if (r == s) v += a.l2;
-  if (a.weights) v *= ld<T>(a.weights, r) * ld<T>(a.weights, s);
-  return v;
-}
 
-// V^-1 of the symmetric positive definite V by Gauss-Jordan without pivoting,
+// V^-1 of the symmetric positive definite V (R x R in LDS, left intact) by
+// Gauss-Jordan without pivoting,
 // held in registers: wave w owns rows w, w + 4, ... of [V | I], lane l owns
 // columns l and l + 64.  Per elimination step the owner of the pivot row
 // publishes it through LDS (double-buffered, one barrier per step) and every
@@ -132,9 +129,8 @@ __device__ __forceinline__ double v_entry(const SolveArgs& a, int r, int s) {
 // the pivots are the (positive) Schur complement diagonals; a pivot that is
 // not positive and finite returns false (the caller falls back to partial
 // pivoting).  Needs blockDim.x == 256 and R <= 64.
-template <typename T>
-__device__ bool spd_gj_inverse(const SolveArgs& a, double* Vi, double* pbuf, int* flag) {
-  const int R = a.R, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+__device__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf, int* flag) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nq = (R - w + 3) >> 2;
   double x[16][2];
 #pragma unroll
@@ -144,7 +140,7 @@ __device__ bool spd_gj_inverse(const SolveArgs& a, double* Vi, double* pbuf, int
     for (int h = 0; h < 2; ++h) {
       const int c = lane + 64 * h;
       double v = 0.0;
-      if (q < nq) v = (c < R) ? v_entry<T>(a, i, c) : ((c - R == i) ? 1.0 : 0.0);
+      if (q < nq) v = (c < R) ? V[i * R + c] : ((c - R == i) ? 1.0 : 0.0);
       x[q][h] = v;
     }
   }
@@ -266,17 +262,16 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __shared__ double Lbuf[kMaxRank * kMaxRank];
   __shared__ double Vi[kMaxRank * kMaxRank];
   __shared__ double Mr[kSolveRows * kMaxRank];
+  __shared__ double pbuf[256];
   __shared__ int flag, piv, ticket, pflag[2];
   const int R = a.R, tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
   const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
   // this block's MTTKRP rows, staged once (coalesced)
   for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
+  build_v<T>(a, Lbuf);  // V from the Grams, all threads (global loads in parallel)
   int status = 0;
-  if (!spd_gj_inverse<T>(a, Vi, Lbuf, pflag)) {
-    build_v<T>(a, Lbuf);
-    status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
-  }
+  if (!spd_gj_inverse(Lbuf, R, Vi, pbuf, pflag)) status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
   if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
   __syncthreads();
   // New rows: A[i, s] = sum_r M[i, r] Vi[r, s]  (V symmetric); keep the
@@ -332,10 +327,36 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
       s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     a.gram_out[e] = s;
   }
-  if (a.ip_partial && tid == 0) {
+  double mn2 = 0.0;  // ||[[w; A]]||^2 = sum (Hadamard of all Grams) w w^T, for the fused error
+  for (int e = tid; e < R * R; e += blockDim.x) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += __hip_atomic_load(a.ip_partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *a.iprod_out = s;
+    for (int b = 0; b < nblk; ++b)
+      s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a.gram_out[e] = s;
+    if (a.errors) {
+      double v = s;
+      for (int m = 0; m < a.ndim; ++m)
+        if (m != a.mode) v *= a.grams[m][e];
+      if (a.weights) v *= ld<T>(a.weights, e / R) * ld<T>(a.weights, e % R);
+      mn2 += v;
+    }
+  }
+  double ip_all = 0.0;
+  if (a.ip_partial && tid == 0) {
+    for (int b = 0; b < nblk; ++b)
+      ip_all += __hip_atomic_load(a.ip_partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *a.iprod_out = ip_all;
+  }
+  if (a.errors) {
+    __shared__ double red2[4];
+    mn2 = wave_sum(mn2);
+    if ((tid & 63) == 0) red2[tid >> 6] = mn2;
+    __syncthreads();
+    if (tid == 0) {
+      const double m2 = (red2[0] + red2[1]) + (red2[2] + red2[3]);
+      const double nx2 = *a.normx2;
+      a.errors[a.err_slot] = sqrt(fabs(nx2 + m2 - 2.0 * ip_all)) / sqrt(nx2);
+    }
   }
   if (tid == 0) *a.counter = 0;
 }
@@ -422,10 +443,10 @@ extern "C" size_t tla_cp_solve_workspace_size(int64_t rows, int rank) {
   return tla_gram64_workspace_size(rows, rank);
 }
 
-extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int mode,
-                            const void* weights, double l2_reg, const void* mttkrp, int64_t rows,
-                            int rank, void* factor_out, double* gram_out, double* iprod_out,
-                            int* status_out, int* counter, void* ws, size_t ws_bytes, void* stream) {
+int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights,
+                         double l2_reg, const void* mttkrp, int64_t rows, int rank, void* factor_out,
+                         double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
+                         size_t ws_bytes, const double* normx2, double* errors, int err_slot, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_solve: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_solve: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_solve: mode %d", mode);
@@ -447,6 +468,10 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
   a.counter = counter;
   a.gram_out = gram_out;
   a.iprod_out = iprod_out;
+  TLA_REQUIRE(!errors || (normx2 && iprod_out && counter), "cp_solve: the fused error needs normx2, iprod, counter");
+  a.normx2 = normx2;
+  a.errors = errors;
+  a.err_slot = err_slot;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TLA_F32) cp_solve_kernel<float><<<nblk, 256, 0, st>>>(a);
   else cp_solve_kernel<double><<<nblk, 256, 0, st>>>(a);
@@ -472,4 +497,12 @@ extern "C" int tla_cp_error(const double* const* grams, int ndim, const void* we
   else cp_error_kernel<double><<<1, 256, 0, st>>>(a, normx2, iprod, errors, slot);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
+}
+
+extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int mode, const void* weights,
+                            double l2_reg, const void* mttkrp, int64_t rows, int rank, void* factor_out,
+                            double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
+                            size_t ws_bytes, void* stream) {
+  return tla::cp_solve_launch(dtype, grams, ndim, mode, weights, l2_reg, mttkrp, rows, rank, factor_out, gram_out,
+                              iprod_out, status_out, counter, ws, ws_bytes, nullptr, nullptr, 0, stream);
 }

@@ -58,6 +58,7 @@ struct BatchArgs {
   const float* probe_alphas;
   int n_probe;
   int max_dec;
+  int first_dec;  // probing covers decisions first_dec .. first_dec + max_dec - 1
   double* hist_err;  // [P][max_sweeps]
   float* hist_alpha;
   int* hist_acc;
@@ -535,8 +536,9 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
       __syncthreads();
       float alpha = a.alpha_fixed;
       store_factors<NW>(p.save);
-      if (probe && dec < a.max_dec) {
-        if (threadIdx.x < kMlpIn) a.feats[((int64_t)pid * a.max_dec + dec) * kMlpIn + threadIdx.x] = sm.feat[threadIdx.x];
+      const int d = dec - a.first_dec;
+      if (probe && d >= 0 && d < a.max_dec) {
+        if (threadIdx.x < kMlpIn) a.feats[((int64_t)pid * a.max_dec + d) * kMlpIn + threadIdx.x] = sm.feat[threadIdx.x];
         int best = 0;
         double best_e = 0.0;
         for (int g = 0; g < a.n_probe; ++g) {
@@ -545,7 +547,7 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
           double eg = sweep<NW>(normx2);
           passes += 3;
           if (sweep_status(eg)) eg = 1e300;
-          if (lead) a.probe_err[((int64_t)pid * a.max_dec + dec) * a.n_probe + g] = eg;
+          if (lead) a.probe_err[((int64_t)pid * a.max_dec + d) * a.n_probe + g] = eg;
           if (g == 0 || eg < best_e) {
             best_e = eg;
             best = g;
@@ -553,13 +555,13 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
           load_factors<NW>(p.save);
         }
         alpha = a.probe_alphas[best];
-        ++dec;
-      } else if (a.mlp && !probe) {
+      } else if (a.mlp) {
         const float y = mlp_alpha_block(a.mlp, sm.feat, sm.h1, sm.hpart, a.alpha_max);
         if (lead) sm.iscal[0] = __float_as_int(y);
         __syncthreads();
         alpha = __int_as_float(sm.iscal[0]);
       }
+      ++dec;
       extrapolate<NW>(p.prev, alpha);
       grams_all<NW>();
       const double e2 = sweep<NW>(normx2);
@@ -635,7 +637,7 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
                                   const int* ranks, float* factors, const int64_t* f_off, float* f_prev, float* f_save,
                                   const int* order, int* queue, int policy, int max_sweeps, double tol, int warmup,
                                   const float* mlp_params, float alpha_fixed, float alpha_max,
-                                  const float* probe_alphas, int n_probe, int max_decisions, double* hist_err,
+                                  const float* probe_alphas, int n_probe, int max_decisions, int first_decision, double* hist_err,
                                   float* hist_alpha, int* hist_acc, float* features, double* probe_err,
                                   int* sweeps_out, int* passes_out, double* err_out, int* n_extrap_out,
                                   int* n_accept_out, int* status_out, long long* ticks_out, int waves, int grid,
@@ -655,7 +657,8 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
     TLA_REQUIRE(warmup >= 1, "batched_cp: warmup = %d", warmup);
   }
   if (policy == kProbe) {
-    TLA_REQUIRE(probe_alphas && features && probe_err && n_probe >= 1 && n_probe <= kMaxProbe && max_decisions >= 1,
+    TLA_REQUIRE(probe_alphas && features && probe_err && n_probe >= 1 && n_probe <= kMaxProbe && max_decisions >= 1 &&
+                    first_decision >= 0,
                 "batched_cp: probing needs 1..%d step values and the feature / probe buffers", kMaxProbe);
   }
   BatchArgs a;
@@ -680,6 +683,7 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
   a.probe_alphas = probe_alphas;
   a.n_probe = n_probe;
   a.max_dec = max_decisions;
+  a.first_dec = first_decision;
   a.hist_err = hist_err;
   a.hist_alpha = hist_alpha;
   a.hist_acc = hist_acc;

@@ -82,6 +82,13 @@ inline void strided_reduce(const float* in, int64_t groups, int64_t E, int64_t K
   strided_reduce_kernel<<<grid, 256, 0, st>>>(in, E, K, gs, ks, out);
 }
 
+// tla_cp_solve with, optionally, the relative error of the updated model
+// written to errors[err_slot] by the finalising block (last mode of a sweep).
+int cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights, double l2_reg,
+                    const void* mttkrp, int64_t rows, int rank, void* factor_out, double* gram_out, double* iprod_out,
+                    int* status_out, int* counter, void* ws, size_t ws_bytes, const double* normx2, double* errors,
+                    int err_slot, void* stream);
+
 struct ModeShape {
   int ndim;
   int64_t shape[kMaxOrder];

@@ -72,6 +72,8 @@ struct CpAls {
   double* scales;
   int* status;
   int* counter;
+  double* err_out;  // errors[] of the current sweep (or NULL): fused into the last solve
+  int err_slot;
   float* M;
   float* T1;
   float* T2;
@@ -139,8 +141,9 @@ static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, hipStrea
   const double* gp[kMaxOrder];
   for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
-  return tla_cp_solve(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n],
-                      h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws, h->ws_bytes, st);
+  return cp_solve_launch(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n], h->grams[n],
+                         want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws, h->ws_bytes, h->normx2,
+                         want_iprod ? h->err_out : nullptr, h->err_slot, st);
 }
 
 static int mttkrp_mode(CpAls* h, int n, hipStream_t st) {
@@ -313,12 +316,11 @@ extern "C" int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int
   for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
   for (int s = 0; s < n_sweeps; ++s) {
+    h->err_out = errors;  // the last mode's solve writes errors[first_slot + s]
+    h->err_slot = first_slot + s;
     int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
+    h->err_out = nullptr;
     if (rc) return rc;
-    if (errors) {
-      rc = tla_cp_error(gp, h->ndim, h->weights, dt, h->R, h->normx2, h->iprod, errors, first_slot + s, st);
-      if (rc) return rc;
-    }
     if (normalize) {
       TLA_REQUIRE(dt == TLA_F32, "cp_als: normalize_factors supports float32");
       GramPtrs g{};
@@ -563,9 +565,11 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
   const dim3 fgrid((unsigned)ceil_div(maxn, 256), (unsigned)h->ndim);
   const dim3 egrid((unsigned)maxblk, (unsigned)h->ndim);
   auto sweep = [&](int slot) -> int {
-    int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
-    if (rc) return rc;
-    return tla_cp_error(gp, h->ndim, h->weights, TLA_F32, h->R, h->normx2, h->iprod, errors, slot, st);
+    h->err_out = errors;
+    h->err_slot = slot;
+    const int rc = h->dimtree ? sweep_dimtree(h, st) : sweep_standard(h, st);
+    h->err_out = nullptr;
+    return rc;
   };
   int s = 0;
   // plain warm-up sweeps (flags/alphas for these slots are set by the caller)

@@ -340,6 +340,97 @@ __global__ __launch_bounds__(256) void rankdiag_axis0_kernel(const float* __rest
   if (il == 0 && e < JR) out[e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
 }
 
+// Vectorised forms for rank % 4 == 0 and 16-byte aligned operands.
+// axis 1: one block per i; lane (row lane jl, column quad c) accumulates
+// Y[i, j, 4c..4c+3] * F[j, 4c..4c+3] over rows j = jl, jl + L, ... with two
+// independent float4 sums (loads of four rows in flight), then the L row
+// lanes are summed in order.
+__global__ __launch_bounds__(256) void rankdiag_axis1_v4_kernel(const float* __restrict__ Y, int64_t J, int R,
+                                                                const float* __restrict__ F,
+                                                                float* __restrict__ out) {
+  const int64_t i = blockIdx.x;
+  __shared__ float4 red[256];
+  const int C = R >> 2;
+  const int L = 256 / C;
+  const int c = threadIdx.x % C, jl = threadIdx.x / C;
+  float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+  if (jl < L) {
+    const float4* y = reinterpret_cast<const float4*>(Y + i * J * R) + c;
+    const float4* f = reinterpret_cast<const float4*>(F) + c;
+    int64_t j = jl;
+    for (; j + 3 * L < J; j += 4 * L) {
+      const float4 y0 = y[j * C], y1 = y[(j + L) * C], y2 = y[(j + 2 * L) * C], y3 = y[(j + 3 * L) * C];
+      const float4 f0 = f[j * C], f1 = f[(j + L) * C], f2 = f[(j + 2 * L) * C], f3 = f[(j + 3 * L) * C];
+      s0.x = fmaf(y0.x, f0.x, s0.x); s0.y = fmaf(y0.y, f0.y, s0.y); s0.z = fmaf(y0.z, f0.z, s0.z); s0.w = fmaf(y0.w, f0.w, s0.w);
+      s1.x = fmaf(y1.x, f1.x, s1.x); s1.y = fmaf(y1.y, f1.y, s1.y); s1.z = fmaf(y1.z, f1.z, s1.z); s1.w = fmaf(y1.w, f1.w, s1.w);
+      s0.x = fmaf(y2.x, f2.x, s0.x); s0.y = fmaf(y2.y, f2.y, s0.y); s0.z = fmaf(y2.z, f2.z, s0.z); s0.w = fmaf(y2.w, f2.w, s0.w);
+      s1.x = fmaf(y3.x, f3.x, s1.x); s1.y = fmaf(y3.y, f3.y, s1.y); s1.z = fmaf(y3.z, f3.z, s1.z); s1.w = fmaf(y3.w, f3.w, s1.w);
+    }
+    for (; j < J; j += L) {
+      const float4 y0 = y[j * C], f0 = f[j * C];
+      s0.x = fmaf(y0.x, f0.x, s0.x); s0.y = fmaf(y0.y, f0.y, s0.y); s0.z = fmaf(y0.z, f0.z, s0.z); s0.w = fmaf(y0.w, f0.w, s0.w);
+    }
+  }
+  red[threadIdx.x] = make_float4(s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w);
+  __syncthreads();
+  if (threadIdx.x < R) {
+    const int cc = threadIdx.x >> 2, comp = threadIdx.x & 3;
+    float t = 0.0f;
+    for (int k = 0; k < L; ++k) {
+      const float4 v = red[k * C + cc];
+      t += comp == 0 ? v.x : (comp == 1 ? v.y : (comp == 2 ? v.z : v.w));
+    }
+    out[i * R + threadIdx.x] = t;
+  }
+}
+
+// axis 0, first stage: blockIdx.y = split s of the i range; block = 4 i-lanes
+// x 64 float4 lanes over (j, r); partial[s][j * R + r] (second stage: a
+// fixed-order strided reduction over the splits).
+__global__ __launch_bounds__(256) void rankdiag_axis0_v4_kernel(const float* __restrict__ Y, int64_t I, int64_t J,
+                                                                int R, int64_t i_per_split,
+                                                                const float* __restrict__ F,
+                                                                float* __restrict__ partial) {
+  __shared__ float4 red[4][64];
+  const int el = threadIdx.x & 63, il = threadIdx.x >> 6;
+  const int64_t JR4 = J * R / 4;
+  const int64_t e4 = (int64_t)blockIdx.x * 64 + el;
+  const int64_t i0 = (int64_t)blockIdx.y * i_per_split;
+  const int64_t i1 = min(I, i0 + i_per_split);
+  float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+  if (e4 < JR4) {
+    const int c = (int)(e4 % (R / 4));
+    const float4* y = reinterpret_cast<const float4*>(Y) + e4;
+    const float4* f = reinterpret_cast<const float4*>(F) + c;
+    int64_t i = i0 + il;
+    for (; i + 4 < i1; i += 8) {
+      const float4 y0 = y[i * JR4], y1 = y[(i + 4) * JR4];
+      const float4 f0 = f[i * (R / 4)], f1 = f[(i + 4) * (R / 4)];
+      s0.x = fmaf(y0.x, f0.x, s0.x); s0.y = fmaf(y0.y, f0.y, s0.y); s0.z = fmaf(y0.z, f0.z, s0.z); s0.w = fmaf(y0.w, f0.w, s0.w);
+      s1.x = fmaf(y1.x, f1.x, s1.x); s1.y = fmaf(y1.y, f1.y, s1.y); s1.z = fmaf(y1.z, f1.z, s1.z); s1.w = fmaf(y1.w, f1.w, s1.w);
+    }
+    for (; i < i1; i += 4) {
+      const float4 y0 = y[i * JR4], f0 = f[i * (R / 4)];
+      s0.x = fmaf(y0.x, f0.x, s0.x); s0.y = fmaf(y0.y, f0.y, s0.y); s0.z = fmaf(y0.z, f0.z, s0.z); s0.w = fmaf(y0.w, f0.w, s0.w);
+    }
+  }
+  red[il][el] = make_float4(s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w);
+  __syncthreads();
+  if (il == 0 && e4 < JR4) {
+    const float4 a = red[0][el], b = red[1][el], c2 = red[2][el], d = red[3][el];
+    reinterpret_cast<float4*>(partial + (int64_t)blockIdx.y * J * R)[e4] =
+        make_float4((a.x + b.x) + (c2.x + d.x), (a.y + b.y) + (c2.y + d.y), (a.z + b.z) + (c2.z + d.z),
+                    (a.w + b.w) + (c2.w + d.w));
+  }
+}
+
+static int64_t axis0_splits(int64_t I, int64_t J, int R) {
+  const int64_t tiles = (J * R / 4 + 63) / 64;
+  int64_t S = (1024 + tiles - 1) / tiles;
+  S = std::max<int64_t>(1, std::min<int64_t>(S, I / 8));
+  return std::min<int64_t>(S, 64);
+}
+
 __global__ void sum_splits_kernel(const float* __restrict__ partial, int64_t n, int splits,
                                   float* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -455,8 +546,9 @@ extern "C" int tla_mode_dot(int dtype, const void* X, int ndim, const int64_t* s
 }
 
 extern "C" size_t tla_rankdiag_contract_workspace_size(int64_t I, int64_t J, int rank, int axis) {
-  (void)I; (void)J; (void)rank; (void)axis;
-  return 0;
+  if (axis != 0 || rank % 4 != 0) return 0;
+  const int64_t S = axis0_splits(I, J, rank);
+  return S > 1 ? align256(size_t(S) * J * rank * 4) : 0;
 }
 
 extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const float* F, int axis,
@@ -465,13 +557,27 @@ extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int r
   TLA_REQUIRE(axis == 0 || axis == 1, "rankdiag: axis %d", axis);
   TLA_REQUIRE(ws_bytes >= tla_rankdiag_contract_workspace_size(I, J, rank, axis), "rankdiag: workspace too small");
   hipStream_t st = (hipStream_t)stream;
+  const bool v4 = rank % 4 == 0 && rank <= 64 && (((uintptr_t)Y | (uintptr_t)F | (uintptr_t)out) & 15) == 0;
   if (axis == 1) {
     TLA_REQUIRE(I <= 0x7fffffff, "rankdiag: I too large");
-    rankdiag_axis1_kernel<<<(unsigned)I, 256, 0, st>>>(Y, J, rank, F, out);
+    if (v4)
+      rankdiag_axis1_v4_kernel<<<(unsigned)I, 256, 0, st>>>(Y, J, rank, F, out);
+    else
+      rankdiag_axis1_kernel<<<(unsigned)I, 256, 0, st>>>(Y, J, rank, F, out);
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
   const int64_t n = J * rank;
+  const int64_t S = v4 ? axis0_splits(I, J, rank) : 1;
+  if (v4 && S > 1 && ws && (((uintptr_t)ws) & 15) == 0) {
+    const int64_t per = (I + S - 1) / S;
+    const dim3 grid((unsigned)ceil_div(n / 4, 64), (unsigned)S);
+    rankdiag_axis0_v4_kernel<<<grid, 256, 0, st>>>(Y, I, J, rank, per, F, (float*)ws);
+    TLA_LAUNCH_CHECK();
+    strided_reduce((const float*)ws, 1, n, S, 0, n, out, st);
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
   rankdiag_axis0_kernel<<<(unsigned)ceil_div(n, 64), 256, 0, st>>>(Y, I, J, rank, F, out);
   TLA_LAUNCH_CHECK();
   return TLA_OK;

@@ -18,7 +18,6 @@ sweep, record its error) and the best step is the regression target
 ``log2(1 + alpha*)``.  Training runs on the GPU with the hand-written
 forward / backward / Adam kernel (``tla_mlp_train``).
 """
-import ctypes
 import os
 
 import numpy as np
@@ -121,7 +120,7 @@ class LearnedExtrapolation:
     """Extrapolation policy for :func:`tensorly_amd.parafac`: a trained
     :class:`ExtrapolationMLP`, or ``model=None`` with a fixed step."""
 
-    def __init__(self, model=None, alpha_fixed=1.0, alpha_max=8.0, warmup=3):
+    def __init__(self, model=None, alpha_fixed=1.0, alpha_max=16.0, warmup=3):
         self.model = model
         self.alpha_fixed = float(alpha_fixed)
         self.alpha_max = float(alpha_max)

@@ -156,6 +156,24 @@ def test_partial_mttkrp_and_rankdiag(cuda, shape, prec):
                 assert rel(rankdiag_contract(T2.contiguous(), F, axis).cpu(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("I,J,R", [(512, 48, 32), (300, 20, 8), (3, 5, 12), (64, 33, 7), (257, 64, 64)])
+def test_rankdiag_contract_against_einsum(cuda, I, J, R):
+    """Both contraction axes, vectorised (rank % 4 == 0, split reduction)
+    and scalar forms, against the FP64 einsum."""
+    from tensorly_amd.tenalg import rankdiag_contract
+    rng = np.random.default_rng(I * 131 + J * 7 + R)
+    Y = rng.standard_normal((I, J, R)).astype(np.float32)
+    FJ = rng.standard_normal((J, R)).astype(np.float32)
+    FI = rng.standard_normal((I, R)).astype(np.float32)
+    Yg = g(Y, cuda, torch.float32)
+    out1 = rankdiag_contract(Yg, g(FJ, cuda, torch.float32), 1).cpu().numpy()
+    out0 = rankdiag_contract(Yg, g(FI, cuda, torch.float32), 0).cpu().numpy()
+    ref1 = np.einsum("ijr,jr->ir", Y.astype(np.float64), FJ.astype(np.float64))
+    ref0 = np.einsum("ijr,ir->jr", Y.astype(np.float64), FI.astype(np.float64))
+    assert rel(out1, ref1) < 1e-5
+    assert rel(out0, ref0) < 1e-5
+
+
 def test_mode_dot_known_answers(cuda, known_answers):
     import tensorly_amd as tla
     ka = known_answers["mode_dot_matrix"]
