@@ -55,6 +55,11 @@ def parse():
     p.add_argument("--quick", action="store_true", help="skip time-to-fit and the CPU baseline")
     p.add_argument("--fit-sweeps", type=int, default=100)
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    p.add_argument("--kernel-only", choices=["partial01", "last", "mode0", "copy"],
+                   help="only launch one kernel --iters times on the 512^3 data (for counter runs)")
+    p.add_argument("--iters", type=int, default=5)
+    p.add_argument("--study", choices=["precision", "tucker"],
+                   help="run the 128^4 rank-64 precision study or the 256^3 Tucker study instead")
     return p.parse_args()
 
 
@@ -94,6 +99,162 @@ def time_to_fit(engine, n_sweeps, target_err, torch):
         return None, None, e
     s = int(hit[0])
     return evs[0].elapsed_time(evs[s + 1]) / 1e3, s + 1, e
+
+
+def kernel_only(args, torch, dev):
+    """Launch one kernel --iters times on the 512^3 workload (bf16x3 for the
+    MTTKRP kernels) and nothing else, so per-dispatch hardware counters can be
+    read without attribution; 'copy' is a plain 512 MiB device copy used to
+    calibrate the counters' byte scale."""
+    import tensorly_amd as tla
+    from tensorly_amd.tenalg import mttkrp_partial
+    torch.manual_seed(0)
+    X = torch.rand(SHAPE, device=dev)
+    f = [torch.rand((s, RANK), device=dev) for s in SHAPE]
+    if args.kernel_only == "copy":
+        Y = torch.empty_like(X)
+        fn = lambda: Y.copy_(X)  # noqa: E731
+    elif args.kernel_only == "partial01":
+        fn = lambda: mttkrp_partial(X, (None, f), (0, 2))  # noqa: E731
+    else:
+        mode = 2 if args.kernel_only == "last" else 0
+        fn = lambda: tla.unfolding_dot_khatri_rao(X, (None, f), mode)  # noqa: E731
+    torch.cuda.synchronize()
+    with tla.precision("bf16x3"):
+        for _ in range(args.iters):
+            fn()
+    torch.cuda.synchronize()
+    log(f"[bench] kernel-only {args.kernel_only}: {args.iters} launches")
+
+
+def _pow2_column_scales(F):
+    """Power-of-two column scales bringing each column's max |entry| into
+    [0.5, 1): exact in binary floating point."""
+    import torch
+    m = F.abs().amax(dim=0).clamp_min(1e-30)
+    return torch.exp2(-torch.ceil(torch.log2(m)))
+
+
+def precision_study(torch, dev):
+    """128^4 float32 (rank-64 CP model, unit N(0,1) columns, weights
+    log-uniform in [1e-2, 1], 0.1 % noise): per-mode MTTKRP relative error
+    against the FP64 numpy oracle for FP32, BF16 and bf16x3 (with and
+    without power-of-two column scaling), then the ALS fit after 200 sweeps
+    in each precision next to an FP64 GPU run."""
+    import tensorly_amd as tla
+    from oracle import tensorly_np as T
+    from tensorly_amd import synthetic
+    from tensorly_amd.decomposition._cp import CpAlsEngine
+    t0 = time.time()
+    x, (w_true, _) = synthetic.cp_128x4_rank64()
+    R = 64
+    log(f"[study] 128^4 data in {time.time() - t0:.1f}s")
+    rng = np.random.RandomState(2)
+    init = [rng.random_sample((128, R)) for _ in range(4)]
+    X = torch.from_numpy(x).to(dev)
+    Xb = X.to(torch.bfloat16)
+    fg = [torch.tensor(f, dtype=torch.float32, device=dev) for f in init]
+    x64 = x.astype(np.float64)
+    out = {"workload": "128^4 float32 CP model, rank 64, unit N(0,1) columns, weights 10**U(-2,0), 0.1% noise",
+           "mttkrp_rel_error": {}, "column_scaling_max_abs_change": {}}
+    for mode in range(4):
+        t1 = time.time()
+        ref = T.unfolding_dot_khatri_rao(x64, (None, [f.astype(np.float32).astype(np.float64) for f in init]), mode)
+        log(f"[study] FP64 oracle MTTKRP mode {mode} in {time.time() - t1:.1f}s")
+        errs = {}
+        for prec in ("fp32", "bf16", "bf16x3"):
+            src = Xb if prec == "bf16" else X
+            with tla.precision(prec):
+                m = tla.unfolding_dot_khatri_rao(src, (None, fg), mode).double().cpu().numpy()
+                # per-mode column scaling: scale factor columns by powers of two, undo on the output
+                sc = [_pow2_column_scales(f) for f in fg]
+                m2 = tla.unfolding_dot_khatri_rao(src, (None, [f * s for f, s in zip(fg, sc)]), mode)
+                undo = torch.ones(R, device=dev)
+                for k in range(4):
+                    if k != mode:
+                        undo = undo * sc[k]
+                m2 = (m2 / undo).double().cpu().numpy()
+            errs[prec] = float(np.linalg.norm(m - ref) / np.linalg.norm(ref))
+            out["column_scaling_max_abs_change"].setdefault(prec, 0.0)
+            out["column_scaling_max_abs_change"][prec] = max(out["column_scaling_max_abs_change"][prec],
+                                                              float(np.max(np.abs(m2 - m))))
+        out["mttkrp_rel_error"][f"mode{mode}"] = errs
+        log(f"[study] mode {mode}: {errs}")
+    del x64
+    fits = {}
+    ones = torch.ones(R, dtype=torch.float32, device=dev)
+    for prec in ("fp32", "bf16", "bf16x3"):
+        eng = CpAlsEngine(X, ones, fg, precision=prec, dimtree=(prec != "fp32"),
+                          tensor_bf16=(Xb if prec == "bf16" else None))
+        errs = torch.zeros(200, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        eng.sweeps(200, errs, 0, False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        e = errs.cpu().numpy()
+        fits[prec] = {"final_error": float(e[-1]), "fit": float(1 - e[-1]), "ms_per_sweep": round(dt / 200 * 1e3, 4)}
+        eng.close()
+        log(f"[study] ALS {prec}: {fits[prec]}")
+    X64 = X.double()
+    eng = CpAlsEngine(X64, ones.double(), [f.double() for f in fg], precision="valu")
+    errs = torch.zeros(200, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.sweeps(200, errs, 0, False)
+    torch.cuda.synchronize()
+    e = errs.cpu().numpy()
+    fits["fp64"] = {"final_error": float(e[-1]), "fit": float(1 - e[-1]),
+                    "ms_per_sweep": round((time.perf_counter() - t1) / 200 * 1e3, 4)}
+    eng.close()
+    for prec in ("fp32", "bf16", "bf16x3"):
+        fits[prec]["fit_minus_fp64"] = fits[prec]["fit"] - fits["fp64"]["fit"]
+    out["als_200_sweeps"] = fits
+    return out
+
+
+def tucker_study(torch, dev):
+    """256^3 float32 Tucker model (32^3 core), 20 HOOI iterations (tol=0,
+    init='svd') on the GPU against the FP64 numpy oracle on the host: time
+    per iteration, TTM throughput, rec. error, core and subspace agreement."""
+    import tensorly_amd as tla
+    from oracle import tensorly_np as T
+    from tensorly_amd import synthetic
+    x, _ = synthetic.tucker_256_core32()
+    X = torch.from_numpy(x).to(dev)
+    rank = [32, 32, 32]
+    tla.tucker(X, rank, n_iter_max=2, init="svd", tol=0)  # warm-up
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    (core, factors), errs = tla.tucker(X, rank, n_iter_max=20, init="svd", tol=0, return_errors=True)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t1
+    log(f"[study] GPU tucker 20 iterations: {gpu_s:.4f}s")
+    from threadpoolctl import threadpool_limits
+    cores = os.cpu_count() or 1
+    with threadpool_limits(cores):
+        t1 = time.perf_counter()
+        (core_r, fac_r), errs_r = T.tucker(x.astype(np.float64), rank, n_iter_max=20, init="svd", tol=0,
+                                           return_errors=True)
+        cpu_s = time.perf_counter() - t1
+    log(f"[study] oracle tucker 20 iterations: {cpu_s:.2f}s on {cores} threads")
+    angles = []
+    for m in range(3):
+        u = factors[m].double().cpu().numpy()
+        s = np.linalg.svd(u.T @ fac_r[m], compute_uv=False)
+        angles.append(float(np.degrees(np.arccos(np.clip(s.min(), -1.0, 1.0)))))
+    core_g = core.double().cpu().numpy()
+    # per iteration: three partial TTM chains (two full-tensor products each) and the core
+    ttm_flops_iter = 3 * (2 * 256 * 32 * 256 * 256 + 2 * 256 * 32 * 32 * 256) + 2 * 256 * 32 * 32 * 32
+    return {"workload": "256^3 float32 Tucker model, core 32^3 N(0,1), orthonormal factors, 1% noise; "
+                        "tucker(rank=[32,32,32], n_iter_max=20, init='svd', tol=0)",
+            "gpu_seconds": gpu_s, "gpu_ms_per_iteration_incl_init": gpu_s / 20 * 1e3,
+            "ttm_gflop_per_iteration": ttm_flops_iter / 1e9,
+            "final_rec_error": float(errs[-1]), "oracle_final_rec_error": float(errs_r[-1]),
+            "core_rel_diff": float(np.linalg.norm(core_g - core_r) / np.linalg.norm(core_r)),
+            "max_subspace_angle_deg": max(angles),
+            "cpu_baseline": {"seconds": cpu_s, "cores": cores, "kind": "port",
+                             "sample": "numpy FP64 restatement, same 20 iterations with init='svd'"}}
 
 
 def load_model(torch, dev):
@@ -233,6 +394,15 @@ def main():
     from tensorly_amd.tenalg import mttkrp_partial
 
     dev = torch.device("cuda:0")
+    if args.kernel_only:
+        kernel_only(args, torch, dev)
+        return
+    if args.study == "precision":
+        print(json.dumps(precision_study(torch, dev)), flush=True)
+        return
+    if args.study == "tucker":
+        print(json.dumps(tucker_study(torch, dev)), flush=True)
+        return
     t0 = time.time()
     x_host, _ = synthetic.cp_512_rank32()
     log(f"[bench] data generated in {time.time() - t0:.1f}s")

@@ -129,7 +129,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // the pivots are the (positive) Schur complement diagonals; a pivot that is
 // not positive and finite returns false (the caller falls back to partial
 // pivoting).  Needs blockDim.x == 256 and R <= 64.
-__device__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf, int* flag) {
+__device__ __forceinline__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf, int* flag) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nq = (R - w + 3) >> 2;
   double x[16][2];
@@ -157,8 +157,9 @@ __device__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf,
         }
       const double piv = readlane_f64(kh ? r1 : r0, kl);
       const bool bad = !(piv > 0.0) || !isfinite(piv);
-      r0 /= piv;
-      r1 /= piv;
+      const double inv = 1.0 / piv;
+      r0 *= inv;
+      r1 *= inv;
 #pragma unroll
       for (int q = 0; q < 16; ++q)
         if (q == qk) {
@@ -172,13 +173,14 @@ __device__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf,
     __syncthreads();
     if (flag[k & 1]) return false;
     const double p0 = buf[lane], p1 = buf[64 + lane];
+    // branch-free over all 16 row slots: rows outside [0, R) hold zeros, the
+    // pivot row gets a zero multiplier (uniform selects, no divergent jumps)
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      if (q < nq && w + 4 * q != k) {
-        const double f = readlane_f64(kh ? x[q][1] : x[q][0], kl);
-        x[q][0] = fma(-f, p0, x[q][0]);
-        x[q][1] = fma(-f, p1, x[q][1]);
-      }
+      const double fr = readlane_f64(kh ? x[q][1] : x[q][0], kl);
+      const double f = (w + 4 * q != k) ? fr : 0.0;
+      x[q][0] = fma(-f, p0, x[q][0]);
+      x[q][1] = fma(-f, p1, x[q][1]);
     }
   }
 #pragma unroll
@@ -192,6 +194,41 @@ __device__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf,
   }
   __syncthreads();
   return true;
+}
+
+// Same elimination for R <= 32 inside ONE wave (lane c owns column c of
+// [V | I], rows in registers): no barriers and no LDS traffic per pivot, which
+// makes the R sequential steps cheap.  Called by wave 0 only; Vi and the
+// return value are valid after the caller's next barrier / in wave 0.
+__device__ __forceinline__ bool wave_spd_inverse32(const double* V, int R, double* Vi) {
+  const int lane = threadIdx.x & 63;
+  double x[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    double v = 0.0;
+    if (i < R) v = (lane < R) ? V[i * R + lane] : ((lane - R == i) ? 1.0 : 0.0);
+    x[i] = v;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < R) {
+      const double piv = readlane_f64(x[k], k);
+      ok = ok && (piv > 0.0) && isfinite(piv);
+      x[k] *= 1.0 / piv;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        if (i != k) {
+          const double f = readlane_f64(x[i], k);
+          x[i] = fma(-f, x[k], x[i]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    if (i < R && lane >= R && lane < 2 * R) Vi[i * R + (lane - R)] = x[i];
+  return ok;
 }
 
 // Gauss-Jordan inverse with partial pivoting (fallback for V that is not
@@ -271,7 +308,18 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
   build_v<T>(a, Lbuf);  // V from the Grams, all threads (global loads in parallel)
   int status = 0;
-  if (!spd_gj_inverse(Lbuf, R, Vi, pbuf, pflag)) status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
+  bool spd_ok;
+  if (R <= 32) {
+    if (tid < 64) {
+      const bool ok = wave_spd_inverse32(Lbuf, R, Vi);
+      if (tid == 0) pflag[0] = ok ? 1 : 0;
+    }
+    __syncthreads();
+    spd_ok = pflag[0] != 0;
+  } else {
+    spd_ok = spd_gj_inverse(Lbuf, R, Vi, pbuf, pflag);
+  }
+  if (!spd_ok) status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
   if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
   __syncthreads();
   // New rows: A[i, s] = sum_r M[i, r] Vi[r, s]  (V symmetric); keep the
@@ -327,24 +375,51 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
       s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     a.gram_out[e] = s;
   }
+  // Sum the per-block partials in block order.  Every block's partials were
+  // published before its ticket (release fence above, acquire fence here);
+  // this block never read them before, so plain loads see them.  Loads are
+  // issued eight blocks at a time so they overlap.
   double mn2 = 0.0;  // ||[[w; A]]||^2 = sum (Hadamard of all Grams) w w^T, for the fused error
-  for (int e = tid; e < R * R; e += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b)
-      s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a.gram_out[e] = s;
-    if (a.errors) {
-      double v = s;
-      for (int m = 0; m < a.ndim; ++m)
-        if (m != a.mode) v *= a.grams[m][e];
-      if (a.weights) v *= ld<T>(a.weights, e / R) * ld<T>(a.weights, e % R);
-      mn2 += v;
+  for (int e0 = tid; e0 < R * R; e0 += 4 * 256) {
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < nblk; b0 += 8) {
+      double v[8][4];
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = e0 + j * 256;
+          v[bb][j] = (b0 + bb < nblk && e < R * R) ? a.gram_partial[(int64_t)(b0 + bb) * R * R + e] : 0.0;
+        }
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] += v[bb][j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = e0 + j * 256;
+      if (e < R * R) {
+        a.gram_out[e] = g[j];
+        if (a.errors) {
+          double v = g[j];
+          for (int m = 0; m < a.ndim; ++m)
+            if (m != a.mode) v *= a.grams[m][e];
+          if (a.weights) v *= ld<T>(a.weights, e / R) * ld<T>(a.weights, e % R);
+          mn2 += v;
+        }
+      }
     }
   }
   double ip_all = 0.0;
   if (a.ip_partial && tid == 0) {
-    for (int b = 0; b < nblk; ++b)
-      ip_all += __hip_atomic_load(a.ip_partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b0 = 0; b0 < nblk; b0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb) v[bb] = (b0 + bb < nblk) ? a.ip_partial[b0 + bb] : 0.0;
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb) ip_all += v[bb];
+    }
     *a.iprod_out = ip_all;
   }
   if (a.errors) {

@@ -217,9 +217,9 @@ __device__ void mttkrp_tiles(int n) {
   for (int t = 0; t < T; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   int eo = (e0 + kq) / n_in, ei = (e0 + kq) - ((e0 + kq) / n_in) * n_in;
   const float* X = p.X;
-  for (int e = e0; e < e1; e += 16) {
-    float xv[4][T];
-    float bv[4];
+  // Two chunks of 16 reduction indices in flight: the loads of chunk c + 1 are
+  // issued before the MFMAs of chunk c.
+  auto fetch = [&](int e, float (&xv)[4][T], float (&bv)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const bool ok = e + 4 * u + kq < e1;
@@ -233,10 +233,22 @@ __device__ void mttkrp_tiles(int n) {
         ++eo;
       }
     }
+  };
+  float xa[4][T], xb[4][T], ba[4], bb[4];
+  if (e0 < e1) fetch(e0, xa, ba);
+  for (int e = e0; e < e1; e += 32) {
+    const bool more = e + 16 < e1;
+    if (more) fetch(e + 16, xb, bb);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xv[u][t], bv[u], acc[t]);
+      for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xa[u][t], ba[u], acc[t]);
+    if (!more) break;
+    if (e + 32 < e1) fetch(e + 32, xa, ba);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xb[u][t], bb[u], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < T; ++t)
