@@ -58,6 +58,7 @@ def parse():
     p.add_argument("--kernel-only", choices=["partial01", "last", "mode0", "copy"],
                    help="only launch one kernel --iters times on the 512^3 data (for counter runs)")
     p.add_argument("--iters", type=int, default=5)
+    p.add_argument("--model", default=None, help="extrapolation network file (default: the packaged one)")
     p.add_argument("--study", choices=["precision", "tucker"],
                    help="run the 128^4 rank-64 precision study or the 256^3 Tucker study instead")
     return p.parse_args()
@@ -65,6 +66,54 @@ def parse():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "mttkrp_partial01_bf16x3_512cubed_r32_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's readings)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
+def raw_launcher(X, facs, prec, kind, arg, torch):
+    """One C-ABI MTTKRP call with every buffer allocated up front, so the
+    timed region holds only the kernel launch(es)."""
+    from tensorly_amd import _lib
+    L = _lib.lib()
+    pcode = _lib.PRECISIONS[prec]
+    sh, keep_sh = _lib.shape_array(X.shape)
+    if kind == "partial":
+        kb, ke = arg
+        fl = [None if kb <= i < ke else f for i, f in enumerate(facs)]
+        rows = int(np.prod(X.shape[kb:ke]))
+        out = torch.empty((rows, RANK), dtype=torch.float32, device=X.device)
+        nb = L.tla_mttkrp_partial_workspace_size(_lib.F32, 3, sh, RANK, kb, ke, pcode)
+    else:
+        fl = list(facs)
+        out = torch.empty((X.shape[arg], RANK), dtype=torch.float32, device=X.device)
+        nb = L.tla_mttkrp_workspace_size(_lib.F32, 3, sh, RANK, arg, pcode)
+    ws = torch.empty(max(int(nb), 1), dtype=torch.uint8, device=X.device)
+    pp, keep_f = _lib.ptr_array(fl)
+    st = _lib.stream(X.device)
+    xp, op, wp = _lib.ptr(X), _lib.ptr(out), _lib.ptr(ws)
+
+    def fn():
+        if kind == "partial":
+            rc = L.tla_mttkrp_partial(_lib.F32, xp, 3, sh, pp, None, RANK, arg[0], arg[1], pcode, op, wp, nb, st)
+        else:
+            rc = L.tla_mttkrp(_lib.F32, xp, 3, sh, pp, None, RANK, arg, pcode, op, wp, nb, st)
+        _lib.check(rc, kind)
+    fn._keep = (keep_sh, keep_f, out, ws)
+    return fn
 
 
 def kernel_time_ms(fn, iters, torch):
@@ -257,9 +306,9 @@ def tucker_study(torch, dev):
                              "sample": "numpy FP64 restatement, same 20 iterations with init='svd'"}}
 
 
-def load_model(torch, dev):
+def load_model(torch, dev, path=None):
     from tensorly_amd.learned import ExtrapolationMLP, default_model_path
-    path = default_model_path()
+    path = path or default_model_path()
     if not os.path.exists(path):
         return None
     return ExtrapolationMLP(np.load(path)["params"], device=dev)
@@ -272,7 +321,7 @@ def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch):
     from tensorly_amd.decomposition._cp import CpAlsEngine
     from tensorly_amd.learned import LearnedExtrapolation, run_extrapolated_sweeps
     pol = LearnedExtrapolation(net, alpha_max=16.0, warmup=3)
-    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
     errs, acc, alphas, _ = run_extrapolated_sweeps(eng, n_slots, pol)
     e = errs.cpu().numpy()
     a = acc.cpu().numpy()
@@ -284,7 +333,7 @@ def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch):
         out.update({"seconds": None, "sweeps": None})
         return out
     k = int(hit[0]) + 1
-    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -319,8 +368,9 @@ def batched_report(torch, dev, net, log):
     policies = [("plain", {}), ("linesearch", {})]
     if net is not None:
         policies.append(("learned", {"model": net, "alpha_max": 16.0}))
+    hist = {}
     for name, kw in policies:
-        out = b.make_outputs(max_sweeps, history=(name == "learned"))
+        out = b.make_outputs(max_sweeps, history=True)
         f = f0.clone()
         b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)  # warm-up
         f.copy_(f0)
@@ -344,13 +394,31 @@ def batched_report(torch, dev, net, log):
                      "failed": int((out["status"].cpu().numpy() != 0).sum()),
                      "mean_final_error": float(out["error"].cpu().numpy().mean()),
                      "extrapolations": ne, "fallback_rate": (1.0 - na / ne) if ne else None}
+        sw_all = out["sweeps"].cpu().numpy()
+        hist[name] = (out["hist_err"].cpu().numpy(), sw_all, out["error"].cpu().numpy())
         log(f"[bench] batched {name}: {res[name]}")
+    # quality-matched comparison: sweep slots each policy needs to reach the
+    # error at which plain ALS stopped (plain's own stopping sweep for plain)
+    he_p, sw_p, err_p = hist["plain"]
+    for name in hist:
+        he, sw, _ = hist[name]
+        need = []
+        for p in range(len(sw)):
+            e = he[p, :sw[p]]
+            hit = np.nonzero(e <= err_p[p] * (1 + 1e-9))[0]
+            need.append(int(hit[0]) + 1 if len(hit) else int(sw[p]))
+        need = np.asarray(need)
+        res[name]["sweeps_to_plain_final_error"] = {"mean": round(float(need.mean()), 2),
+                                                    "median": float(np.median(need)),
+                                                    "reached": int(sum(
+                                                        1 for p in range(len(sw))
+                                                        if (he[p, :sw[p]] <= err_p[p] * (1 + 1e-9)).any()))}
     return res
 
 
 def cpu_baseline(x32):
-    """One float32 ALS sweep of the numpy restatement, median of 3 after a
-    warm-up."""
+    """Float32 ALS sweeps of the numpy restatement on the host cores: the
+    median over about 10 s of sweeps after a warm-up."""
     from threadpoolctl import threadpool_limits
     from oracle import tensorly_np as T
     cores = os.cpu_count() or 1
@@ -369,16 +437,20 @@ def cpu_baseline(x32):
         T.error_calc(x32, 1.0, w, factors, mttkrp=m)
 
     with threadpool_limits(cores):
-        sweep()
+        sweep()  # warm-up
         ts = []
-        for _ in range(3):
+        t_start = time.perf_counter()
+        while len(ts) < 3 or (time.perf_counter() - t_start < 10.0 and len(ts) < 200):
             t0 = time.perf_counter()
             sweep()
             ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
     return {"value": round(3 * MTTKRP_BYTES / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"one float32 CP-ALS sweep (3 MTTKRPs + solves + fit) on the 512^3 tensor, median of 3 "
-                      f"after 1 warm-up: {t:.3f} s/sweep, numpy {np.__version__}, {platform.processor() or 'cpu'}"}
+            "seconds_per_sweep": t, "time_to_fit_seconds_100_sweeps": 100 * t,
+            "sample": f"float32 CP-ALS sweeps (3 MTTKRPs with unfold copy + materialised Khatri-Rao + np.dot, "
+                      f"Hadamard of Grams, np.linalg.solve, fit) of the numpy restatement on the 512^3 tensor: "
+                      f"median of {len(ts)} sweeps after 1 warm-up ({sum(ts):.1f} s of CPU work), "
+                      f"{t:.3f} s/sweep, numpy {np.__version__}, {platform.processor() or 'cpu'}"}
 
 
 def main():
@@ -391,7 +463,6 @@ def main():
     import tensorly_amd as tla
     from tensorly_amd import synthetic, _lib
     from tensorly_amd.decomposition._cp import CpAlsEngine
-    from tensorly_amd.tenalg import mttkrp_partial
 
     dev = torch.device("cuda:0")
     if args.kernel_only:
@@ -412,7 +483,7 @@ def main():
     ones = torch.ones(RANK, dtype=torch.float32, device=dev)
 
     # ---- timed sweeps: fast path --------------------------------------------------------
-    fast = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True)
+    fast = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
     errs = torch.zeros(max(args.steps, 1), dtype=torch.float64, device=dev)
     fast.sweeps(args.warmup, None, 0, False)
     torch.cuda.synchronize()
@@ -428,24 +499,23 @@ def main():
     # ---- per-kernel roofline (HIP events on the launch stream) -------------------------------
     facs = fast.factors
     kern = {}
-    with tla.precision("bf16x3"):
-        kern["mttkrp_partial_modes01_bf16x3"] = kernel_time_ms(
-            lambda: mttkrp_partial(X, (None, facs), (0, 2)), 20, torch)
-        kern["mttkrp_mode2_bf16x3"] = kernel_time_ms(lambda: tla.unfolding_dot_khatri_rao(X, (None, facs), 2), 20,
-                                                     torch)
-        kern["mttkrp_mode0_bf16x3"] = kernel_time_ms(lambda: tla.unfolding_dot_khatri_rao(X, (None, facs), 0), 20,
-                                                     torch)
-    with tla.precision("fp32"):
-        for m in range(3):
-            kern[f"mttkrp_mode{m}_fp32"] = kernel_time_ms(
-                lambda m=m: tla.unfolding_dot_khatri_rao(X, (None, facs), m), 20, torch)
+    for name, prec, kind, arg in (("mttkrp_partial_modes01_bf16x3", "bf16x3", "partial", (0, 2)),
+                                  ("mttkrp_mode2_bf16x3", "bf16x3", "mttkrp", 2),
+                                  ("mttkrp_mode0_bf16x3", "bf16x3", "mttkrp", 0),
+                                  ("mttkrp_mode0_fp32", "fp32", "mttkrp", 0),
+                                  ("mttkrp_mode1_fp32", "fp32", "mttkrp", 1),
+                                  ("mttkrp_mode2_fp32", "fp32", "mttkrp", 2)):
+        kern[name] = kernel_time_ms(raw_launcher(X, facs, prec, kind, arg, torch), 20, torch)
     partial_bytes = 512 ** 3 * 4 + 512 * RANK * 4 + 512 * 512 * RANK * 4
     kernels = {k: {"ms": round(v, 4),
                    "GB/s": round((partial_bytes if "partial" in k else MTTKRP_BYTES) / (v * 1e-3) / 1e9, 1)}
                for k, v in kern.items()}
-    dom = max(("mttkrp_partial_modes01_bf16x3", "mttkrp_mode2_bf16x3"), key=lambda k: kern[k])
-    dom_bytes = partial_bytes if "partial" in dom else MTTKRP_BYTES
+    # the roofline kernel: the dimension-tree partial contraction is ONE kernel
+    # launch (mttkrp_stream_kernel, row epilogue) and the largest per sweep
+    dom = "mttkrp_partial_modes01_bf16x3"
+    dom_bytes = partial_bytes
     achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
+    traffic = measured_traffic()
     log(f"[bench] kernels: {json.dumps(kernels)}")
 
     # ---- time to fit ----------------------------------------------------------------------------------
@@ -457,12 +527,12 @@ def main():
             target = float(ref[-1]) + 1e-4
             res = {}
             for name, prec, dimtree in (("fp32_plain", "fp32", False), ("bf16x3_dimtree", "bf16x3", True)):
-                eng = CpAlsEngine(X, ones, init, precision=prec, dimtree=dimtree)
+                eng = CpAlsEngine(X, ones, init, precision=prec, dimtree=dimtree, unit_weights=True)
                 t, n, e = time_to_fit(eng, args.fit_sweeps, target, torch)
                 res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
                 eng.close()
                 log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
-            net = load_model(torch, dev)
+            net = load_model(torch, dev, args.model)
             if net is not None:
                 res["bf16x3_dimtree_learned"] = learned_time_to_fit(X, ones, init, net, args.fit_sweeps, target, torch)
                 log(f"[bench] time-to-fit bf16x3_dimtree_learned: {res['bf16x3_dimtree_learned']}")
@@ -478,7 +548,7 @@ def main():
 
     batched = None
     if not (args.quick or args.no_batched):
-        batched = batched_report(torch, dev, load_model(torch, dev), log)
+        batched = batched_report(torch, dev, load_model(torch, dev, args.model), log)
 
     cpu = None
     if not (args.no_cpu or args.quick):
@@ -503,7 +573,8 @@ def main():
                                "one CP-ALS sweep per step (bf16x3 MFMA MTTKRP, dimension tree, FP64 solve+fit)",
                    "rank": RANK, "shape": list(SHAPE), "sweep": "bf16x3+dimtree", "dimtree_active": fast.dimtree},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": dom_bytes},
         "kernels": kernels,
         "time_to_fit": ttf,
         "batched": batched,

@@ -110,6 +110,9 @@ struct SolveArgs {
   const double* normx2;
   double* errors;
   int err_slot;
+  // optional: V^-1 computed ahead (R x R FP64, global) and its status
+  const double* vi_pre;
+  const int* vi_status;
 };
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -294,6 +297,42 @@ __device__ void build_v(const SolveArgs& a, double* V) {
   __syncthreads();
 }
 
+// V (built from the Grams into Vbuf) and V^-1 into Vi, both LDS R x R.
+// Returns 0 (SPD elimination), 1 (partial-pivoting fallback) or 2 (singular).
+template <typename T>
+__device__ int invert_v(const SolveArgs& a, double* Vbuf, double* Vi, double* pbuf, int* pflag, int* piv,
+                        int* flag) {
+  const int R = a.R;
+  build_v<T>(a, Vbuf);  // V from the Grams, all threads (global loads in parallel)
+  bool spd_ok;
+  if (R <= 32) {
+    if (threadIdx.x < 64) {
+      const bool ok = wave_spd_inverse32(Vbuf, R, Vi);
+      if (threadIdx.x == 0) pflag[0] = ok ? 1 : 0;
+    }
+    __syncthreads();
+    spd_ok = pflag[0] != 0;
+  } else {
+    spd_ok = spd_gj_inverse(Vbuf, R, Vi, pbuf, pflag);
+  }
+  if (spd_ok) return 0;
+  return lds_gj_inverse(Vbuf, Vi, R, piv, flag) ? 1 : 2;
+}
+
+// V^-1 alone (one block), so that it can run on a second stream while the
+// MTTKRP of the same mode is still streaming the tensor.
+template <typename T>
+__global__ __launch_bounds__(256) void cp_inverse_kernel(SolveArgs a, double* vi_out, int* status_out) {
+  __shared__ double Vbuf[kMaxRank * kMaxRank];
+  __shared__ double Vi[kMaxRank * kMaxRank];
+  __shared__ double pbuf[256];
+  __shared__ int flag, piv, pflag[2];
+  const int st = invert_v<T>(a, Vbuf, Vi, pbuf, pflag, &piv, &flag);
+  __syncthreads();
+  for (int e = threadIdx.x; e < a.R * a.R; e += blockDim.x) vi_out[e] = Vi[e];
+  if (threadIdx.x == 0) *status_out = st;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __shared__ double Lbuf[kMaxRank * kMaxRank];
@@ -306,20 +345,14 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
   // this block's MTTKRP rows, staged once (coalesced)
   for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
-  build_v<T>(a, Lbuf);  // V from the Grams, all threads (global loads in parallel)
   int status = 0;
-  bool spd_ok;
-  if (R <= 32) {
-    if (tid < 64) {
-      const bool ok = wave_spd_inverse32(Lbuf, R, Vi);
-      if (tid == 0) pflag[0] = ok ? 1 : 0;
-    }
+  if (a.vi_pre) {
+    for (int e = tid; e < R * R; e += blockDim.x) Vi[e] = a.vi_pre[e];
+    status = *a.vi_status;
     __syncthreads();
-    spd_ok = pflag[0] != 0;
   } else {
-    spd_ok = spd_gj_inverse(Lbuf, R, Vi, pbuf, pflag);
+    status = invert_v<T>(a, Lbuf, Vi, pbuf, pflag, &piv, &flag);
   }
-  if (!spd_ok) status = lds_gj_inverse(Lbuf, Vi, R, &piv, &flag) ? 1 : 2;
   if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
   __syncthreads();
   // New rows: A[i, s] = sum_r M[i, r] Vi[r, s]  (V symmetric); keep the
@@ -521,7 +554,8 @@ extern "C" size_t tla_cp_solve_workspace_size(int64_t rows, int rank) {
 int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights,
                          double l2_reg, const void* mttkrp, int64_t rows, int rank, void* factor_out,
                          double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
-                         size_t ws_bytes, const double* normx2, double* errors, int err_slot, void* stream) {
+                         size_t ws_bytes, const double* normx2, double* errors, int err_slot,
+                         const double* vi_pre, const int* vi_status, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_solve: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_solve: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_solve: mode %d", mode);
@@ -547,6 +581,9 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
   a.normx2 = normx2;
   a.errors = errors;
   a.err_slot = err_slot;
+  TLA_REQUIRE(!vi_pre || vi_status, "cp_solve: precomputed inverse without status");
+  a.vi_pre = vi_pre;
+  a.vi_status = vi_status;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TLA_F32) cp_solve_kernel<float><<<nblk, 256, 0, st>>>(a);
   else cp_solve_kernel<double><<<nblk, 256, 0, st>>>(a);
@@ -579,5 +616,27 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
                             double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
                             size_t ws_bytes, void* stream) {
   return tla::cp_solve_launch(dtype, grams, ndim, mode, weights, l2_reg, mttkrp, rows, rank, factor_out, gram_out,
-                              iprod_out, status_out, counter, ws, ws_bytes, nullptr, nullptr, 0, stream);
+                              iprod_out, status_out, counter, ws, ws_bytes, nullptr, nullptr, 0, nullptr, nullptr,
+                              stream);
+}
+
+int tla::cp_inverse_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights,
+                           double l2_reg, int rank, double* vi_out, int* status_out, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_inverse: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder && mode >= 0 && mode < ndim, "cp_inverse: order %d mode %d", ndim,
+              mode);
+  TLA_REQUIRE(rank >= 1 && rank <= kMaxRank, "cp_inverse: rank %d", rank);
+  TLA_REQUIRE(vi_out && status_out, "cp_inverse: NULL output");
+  SolveArgs a{};
+  for (int m = 0; m < ndim; ++m) a.grams[m] = grams[m];
+  a.ndim = ndim;
+  a.mode = mode;
+  a.R = rank;
+  a.weights = weights;
+  a.l2 = l2_reg;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TLA_F32) cp_inverse_kernel<float><<<1, 256, 0, st>>>(a, vi_out, status_out);
+  else cp_inverse_kernel<double><<<1, 256, 0, st>>>(a, vi_out, status_out);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
 }

@@ -87,7 +87,11 @@ inline void strided_reduce(const float* in, int64_t groups, int64_t E, int64_t K
 int cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights, double l2_reg,
                     const void* mttkrp, int64_t rows, int rank, void* factor_out, double* gram_out, double* iprod_out,
                     int* status_out, int* counter, void* ws, size_t ws_bytes, const double* normx2, double* errors,
-                    int err_slot, void* stream);
+                    int err_slot, const double* vi_pre, const int* vi_status, void* stream);
+// V^-1 of the solve for `mode` alone (one workgroup): vi_out R x R FP64,
+// status_out 0 / 1 (pivoting fallback) / 2 (singular).
+int cp_inverse_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights, double l2_reg,
+                      int rank, double* vi_out, int* status_out, void* stream);
 
 struct ModeShape {
   int ndim;

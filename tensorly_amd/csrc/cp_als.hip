@@ -74,6 +74,14 @@ struct CpAls {
   int* counter;
   double* err_out;  // errors[] of the current sweep (or NULL): fused into the last solve
   int err_slot;
+  // V^-1 of a mode is computed on a second stream while that mode's tensor
+  // contraction runs (the Grams it needs are final by then)
+  double* vi[kMaxOrder];
+  int* vist;
+  hipStream_t aux;
+  hipEvent_t ev_fork;
+  hipEvent_t ev_inv[kMaxOrder];
+  bool inv_pending[kMaxOrder];
   float* M;
   float* T1;
   float* T2;
@@ -85,7 +93,7 @@ struct CpAls {
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t grams, normx2, iprod, scales, status, M, T1, T2, ws, total;
+  size_t grams, normx2, iprod, scales, status, vi, vist, M, T1, T2, ws, total;
   size_t ws_bytes;
 };
 
@@ -111,6 +119,8 @@ static Layout layout(int x_dtype, int ndim, const int64_t* shape, int R, int pre
   L.iprod = take(8);
   L.scales = take(size_t(ndim) * R * 8);
   L.status = take(8);
+  L.vi = take(size_t(ndim) * R * R * 8);
+  L.vist = take(size_t(ndim) * 4);
   int64_t maxI = 0;
   int64_t total = 1;
   for (int m = 0; m < ndim; ++m) { maxI = std::max(maxI, shape[m]); total *= shape[m]; }
@@ -137,13 +147,32 @@ static Layout layout(int x_dtype, int ndim, const int64_t* shape, int R, int pre
   return L;
 }
 
+// Start V_n^-1 on the second stream; the Grams it reads are final on `st`
+// at this point.
+static int fork_inverse(CpAls* h, int n, hipStream_t st) {
+  const double* gp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
+  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  TLA_HIP(hipEventRecord(h->ev_fork, st));
+  TLA_HIP(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
+  int rc = cp_inverse_launch(dt, gp, h->ndim, n, h->weights, h->l2, h->R, h->vi[n], h->vist + n, h->aux);
+  if (rc) return rc;
+  TLA_HIP(hipEventRecord(h->ev_inv[n], h->aux));
+  h->inv_pending[n] = true;
+  return TLA_OK;
+}
+
 static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, hipStream_t st) {
   const double* gp[kMaxOrder];
   for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  const bool pre = h->inv_pending[n];
+  if (pre) TLA_HIP(hipStreamWaitEvent(st, h->ev_inv[n], 0));
+  h->inv_pending[n] = false;
   return cp_solve_launch(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n], h->grams[n],
                          want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws, h->ws_bytes, h->normx2,
-                         want_iprod ? h->err_out : nullptr, h->err_slot, st);
+                         want_iprod ? h->err_out : nullptr, h->err_slot, pre ? h->vi[n] : nullptr,
+                         pre ? h->vist + n : nullptr, st);
 }
 
 static int mttkrp_mode(CpAls* h, int n, hipStream_t st) {
@@ -157,7 +186,9 @@ static int sweep_standard(CpAls* h, hipStream_t st) {
   const int N = h->ndim;
   for (int n = 0; n < N; ++n) {
     if (h->fixed[n]) continue;
-    int rc = mttkrp_mode(h, n, st);
+    int rc = fork_inverse(h, n, st);
+    if (rc) return rc;
+    rc = mttkrp_mode(h, n, st);
     if (rc) return rc;
     rc = solve_mode(h, n, h->M, n == N - 1, st);
     if (rc) return rc;
@@ -172,6 +203,10 @@ static int sweep_dimtree(CpAls* h, hipStream_t st) {
   // First half: modes 0 and 1 share T1 = X x_{m>=2} A_m.
   if (!(h->fixed[0] && h->fixed[1])) {
     for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
+    if (!h->fixed[0]) {
+      int rc = fork_inverse(h, 0, st);
+      if (rc) return rc;
+    }
     int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 0, 2, h->precision, h->T1,
                                 h->ws, h->ws_bytes, st);
     if (rc) return rc;
@@ -183,6 +218,9 @@ static int sweep_dimtree(CpAls* h, hipStream_t st) {
       if (rc) return rc;
     }
     if (!h->fixed[1]) {
+      // V_1 needs the Gram of the mode-0 factor just updated; only the short
+      // rank-diagonal contraction could hide it, so it is computed inside the
+      // solve instead of on the second stream
       rc = tla_rankdiag_contract(h->T1, h->shape[0], h->shape[1], R, (const float*)h->factors[0], 0, h->M,
                                  h->ws, h->ws_bytes, st);
       if (rc) return rc;
@@ -191,12 +229,18 @@ static int sweep_dimtree(CpAls* h, hipStream_t st) {
     }
   }
   if (N == 3) {
-    int rc = mttkrp_mode(h, 2, st);
+    int rc = fork_inverse(h, 2, st);
+    if (rc) return rc;
+    rc = mttkrp_mode(h, 2, st);
     if (rc) return rc;
     return solve_mode(h, 2, h->M, true, st);
   }
   // N == 4: modes 2 and 3 share T2 = X x_0 A_0 x_1 A_1 (updated factors).
   for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
+  if (!h->fixed[2]) {
+    int rc = fork_inverse(h, 2, st);
+    if (rc) return rc;
+  }
   int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 2, 4, h->precision, h->T2,
                               h->ws, h->ws_bytes, st);
   if (rc) return rc;
@@ -207,6 +251,8 @@ static int sweep_dimtree(CpAls* h, hipStream_t st) {
     rc = solve_mode(h, 2, h->M, false, st);
     if (rc) return rc;
   }
+  rc = fork_inverse(h, 3, st);
+  if (rc) return rc;
   rc = tla_rankdiag_contract(h->T2, h->shape[2], h->shape[3], R, (const float*)h->factors[2], 0, h->M, h->ws,
                              h->ws_bytes, st);
   if (rc) return rc;
@@ -237,7 +283,9 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   TLA_REQUIRE(x_dtype == TLA_F32 || x_dtype == TLA_F64 || x_dtype == TLA_BF16, "cp_als: bad dtype");
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_als: order %d", ndim);
   TLA_REQUIRE(rank >= 1 && rank <= 64, "cp_als: rank %d not in [1, 64]", rank);
-  TLA_REQUIRE(X && factors && weights && workspace && handle_out, "cp_als: NULL pointer");
+  // weights == NULL: unit weights that stay unit (no normalisation); the
+  // kernels then skip the weight products entirely
+  TLA_REQUIRE(X && factors && workspace && handle_out, "cp_als: NULL pointer");
   if (x_dtype == TLA_F64) precision = TLA_PREC_VALU;
   TLA_REQUIRE(x_dtype != TLA_BF16 || precision == TLA_PREC_BF16, "cp_als: bf16 tensor needs bf16 precision");
   if (dimtree && !dimtree_ok(x_dtype, ndim, shape, rank, precision)) dimtree = 0;
@@ -259,6 +307,16 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   h->scales = (double*)(base + L.scales);
   h->status = (int*)(base + L.status);
   h->counter = h->status + 1;  // the status slot is 8 bytes: int status, int block counter
+  for (int m = 0; m < ndim; ++m) h->vi[m] = (double*)(base + L.vi) + size_t(m) * rank * rank;
+  h->vist = (int*)(base + L.vist);
+  bool ok = hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
+  for (int m = 0; ok && m < ndim; ++m) ok = hipEventCreateWithFlags(&h->ev_inv[m], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    tla_cp_als_destroy(h);
+    set_error("cp_als: could not create the auxiliary stream / events");
+    return TLA_ERR_HIP;
+  }
   h->M = (float*)(base + L.M);
   h->T1 = dimtree ? (float*)(base + L.T1) : nullptr;
   h->T2 = (dimtree && ndim == 4) ? (float*)(base + L.T2) : nullptr;
@@ -271,7 +329,18 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
 
 extern "C" int tla_cp_als_dimtree_active(void* handle) { return handle ? ((CpAls*)handle)->dimtree : 0; }
 
-extern "C" void tla_cp_als_destroy(void* handle) { delete (CpAls*)handle; }
+extern "C" void tla_cp_als_destroy(void* handle) {
+  CpAls* h = (CpAls*)handle;
+  if (!h) return;
+  if (h->aux) {
+    (void)hipStreamSynchronize(h->aux);
+    (void)hipStreamDestroy(h->aux);
+  }
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  for (int m = 0; m < h->ndim; ++m)
+    if (h->ev_inv[m]) (void)hipEventDestroy(h->ev_inv[m]);
+  delete h;
+}
 
 // (Re)compute ‖X‖² and the Grams of the current factors (after the caller
 // changed the factors, e.g. a line-search step).
@@ -325,6 +394,7 @@ extern "C" int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int
       TLA_REQUIRE(dt == TLA_F32, "cp_als: normalize_factors supports float32");
       GramPtrs g{};
       for (int m = 0; m < h->ndim; ++m) g.p[m] = h->grams[m];
+      TLA_REQUIRE(h->weights, "cp_als: normalize_factors needs a weights vector");
       normalize_scales_kernel<<<1, 64, 0, st>>>(g, h->ndim, h->R, h->weights, h->scales);
       TLA_LAUNCH_CHECK();
       for (int m = 0; m < h->ndim; ++m) {
@@ -454,12 +524,28 @@ __global__ __launch_bounds__(256) void extrapolate_gram_kernel(ExtrapArgs a) {
   __syncthreads();
   if (ticket != nblk - 1) return;
   __threadfence();
+  // partials of the other blocks were published before their tickets; this
+  // block never read them, so plain loads (eight blocks in flight) see them
   const double* base = a.gpart + (int64_t)m * a.maxblk * R * R;
-  for (int e = tid; e < R * R; e += blockDim.x) {
-    double acc = 0.0;
-    for (int b = 0; b < nblk; ++b)
-      acc += __hip_atomic_load(base + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a.grams[m][e] = acc;
+  for (int e0 = tid; e0 < R * R; e0 += 4 * 256) {
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < nblk; b0 += 8) {
+      double v[8][4];
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = e0 + j * 256;
+          v[bb][j] = (b0 + bb < nblk && e < R * R) ? base[(int64_t)(b0 + bb) * R * R + e] : 0.0;
+        }
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] += v[bb][j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (e0 + j * 256 < R * R) a.grams[m][e0 + j * 256] = g[j];
   }
   if (tid == 0) a.counters[m] = 0;
 }
@@ -477,12 +563,17 @@ struct GramPtrArray {
   double* p[kMaxOrder];
 };
 
-__global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, GramPtrArray grams, const double* gsave,
-                                        int ndim, int R, double* errors, int* accepted, int s) {
+// A_prev of the next pair is the factor kept here, written in the same pass.
+__global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, FactorPtrs Aprev, GramPtrArray grams,
+                                        const double* gsave, int ndim, int R, double* errors, int* accepted, int s) {
   const bool ok = errors[s + 1] < errors[s];
   const int m = blockIdx.y;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!ok && e < A.n[m]) A.p[m][e] = Asave.p[m][e];
+  if (e < A.n[m]) {
+    const float v = ok ? A.p[m][e] : Asave.p[m][e];
+    if (!ok) A.p[m][e] = v;
+    Aprev.p[m][e] = v;
+  }
   if (blockIdx.x == 0 && blockIdx.y == 0) {
     if (!ok)
       for (int i = threadIdx.x; i < ndim * R * R; i += blockDim.x) grams.p[i / (R * R)][i % (R * R)] = gsave[i];
@@ -580,9 +671,10 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
   // Per pair: save A_prev, plain sweep, step from the network, extrapolate
   // (+ Grams), sweep from there, keep or restore.  The Grams stay current
   // throughout, so no separate Gram pass is needed.
+  // A_prev for the first pair; later pairs get it from accept_or_revert
+  copy_factors_kernel<<<fgrid, 256, 0, st>>>(Aprev, A);
+  TLA_LAUNCH_CHECK();
   while (s < n_sweeps) {
-    copy_factors_kernel<<<fgrid, 256, 0, st>>>(Aprev, A);
-    TLA_LAUNCH_CHECK();
     int rc = sweep(s);
     if (rc) return rc;
     if (s + 1 >= n_sweeps) break;
@@ -593,7 +685,7 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
     extrapolate_gram_kernel<<<egrid, 256, 0, st>>>(ea);
     TLA_LAUNCH_CHECK();
     if ((rc = sweep(s + 1))) return rc;
-    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, gpa, ea.gsave, h->ndim, R, errors, accepted, s);
+    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, Aprev, gpa, ea.gsave, h->ndim, R, errors, accepted, s);
     TLA_LAUNCH_CHECK();
     s += 2;
   }

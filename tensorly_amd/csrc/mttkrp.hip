@@ -1577,9 +1577,13 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     GmArgs g = p.gm;
     for (int m = 0; m < g.n; ++m) g.fac[m] = (const float*)factors[m];
     g.weights = (const float*)weights;
-    const int64_t ne = p.gm_rows * R;
-    build_gm_kernel<<<ceil_div(ne, 256), 256, 0, st>>>(g, gm);
-    TLA_LAUNCH_CHECK();
+    if (g.n == 1 && !weights) {
+      gm = (float*)factors[0];  // the Khatri-Rao row factor is that factor itself
+    } else {
+      const int64_t ne = p.gm_rows * R;
+      build_gm_kernel<<<ceil_div(ne, 256), 256, 0, st>>>(g, gm);
+      TLA_LAUNCH_CHECK();
+    }
     s.X = X;
     s.Fj = (const float*)factors[kb - 1];
     s.Gm = gm;

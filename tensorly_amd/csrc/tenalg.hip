@@ -403,11 +403,20 @@ __global__ __launch_bounds__(256) void rankdiag_axis0_v4_kernel(const float* __r
     const float4* y = reinterpret_cast<const float4*>(Y) + e4;
     const float4* f = reinterpret_cast<const float4*>(F) + c;
     int64_t i = i0 + il;
-    for (; i + 4 < i1; i += 8) {
-      const float4 y0 = y[i * JR4], y1 = y[(i + 4) * JR4];
-      const float4 f0 = f[i * (R / 4)], f1 = f[(i + 4) * (R / 4)];
-      s0.x = fmaf(y0.x, f0.x, s0.x); s0.y = fmaf(y0.y, f0.y, s0.y); s0.z = fmaf(y0.z, f0.z, s0.z); s0.w = fmaf(y0.w, f0.w, s0.w);
-      s1.x = fmaf(y1.x, f1.x, s1.x); s1.y = fmaf(y1.y, f1.y, s1.y); s1.z = fmaf(y1.z, f1.z, s1.z); s1.w = fmaf(y1.w, f1.w, s1.w);
+    for (; i + 28 < i1; i += 32) {  // eight rows in flight per lane
+      float4 yv[8], fv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        yv[u] = y[(i + 4 * u) * JR4];
+        fv[u] = f[(i + 4 * u) * (R / 4)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        s0.x = fmaf(yv[u].x, fv[u].x, s0.x); s0.y = fmaf(yv[u].y, fv[u].y, s0.y);
+        s0.z = fmaf(yv[u].z, fv[u].z, s0.z); s0.w = fmaf(yv[u].w, fv[u].w, s0.w);
+        s1.x = fmaf(yv[u + 1].x, fv[u + 1].x, s1.x); s1.y = fmaf(yv[u + 1].y, fv[u + 1].y, s1.y);
+        s1.z = fmaf(yv[u + 1].z, fv[u + 1].z, s1.z); s1.w = fmaf(yv[u + 1].w, fv[u + 1].w, s1.w);
+      }
     }
     for (; i < i1; i += 4) {
       const float4 y0 = y[i * JR4], f0 = f[i * (R / 4)];
@@ -427,7 +436,7 @@ __global__ __launch_bounds__(256) void rankdiag_axis0_v4_kernel(const float* __r
 static int64_t axis0_splits(int64_t I, int64_t J, int R) {
   const int64_t tiles = (J * R / 4 + 63) / 64;
   int64_t S = (1024 + tiles - 1) / tiles;
-  S = std::max<int64_t>(1, std::min<int64_t>(S, I / 8));
+  S = std::max<int64_t>(1, std::min<int64_t>(S, I / 32));
   return std::min<int64_t>(S, 64);
 }
 

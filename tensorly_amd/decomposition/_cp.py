@@ -130,7 +130,9 @@ class CpAlsEngine:
     buffers (updated in place)."""
 
     def __init__(self, tensor, weights, factors, l2_reg=0.0, precision=None, dimtree=False,
-                 fixed_modes=None, tensor_bf16=None):
+                 fixed_modes=None, tensor_bf16=None, unit_weights=False):
+        """``unit_weights``: the weights are ones and stay ones (no
+        normalisation), so the native driver skips every weight product."""
         self.tensor = tensor
         self.dev = tensor.device
         self.nd = tensor.dim()
@@ -159,7 +161,9 @@ class CpAlsEngine:
         self._fixed = fixed
         fp, self._keep_f = ptr_array(self.factors)
         handle = _lib.c_p()
-        check(lib().tla_cp_als_create(xcode, ptr(self.x), self.nd, self._sh, self.rank, fp, ptr(self.weights),
+        self.unit_weights = bool(unit_weights)
+        wptr = _lib.c_p(0) if self.unit_weights else ptr(self.weights)
+        check(lib().tla_cp_als_create(xcode, ptr(self.x), self.nd, self._sh, self.rank, fp, wptr,
                                       float(l2_reg), pcode, int(self.dimtree), fixed, ptr(self.ws), nb,
                                       _lib.ctypes.byref(handle)), "cp_als_create")
         self.handle = handle
@@ -235,8 +239,9 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", norma
                       "not be fixed. Consider using tl.moveaxis()")
         fixed_modes.remove(nd - 1)
 
+    unit = (not normalize_factors) and bool(torch.all(weights == 1))
     eng = CpAlsEngine(tensor, weights, factors, l2_reg=l2_reg, precision=precision, dimtree=dimtree,
-                      fixed_modes=fixed_modes, tensor_bf16=tensor_bf16)
+                      fixed_modes=fixed_modes, tensor_bf16=tensor_bf16, unit_weights=unit)
     try:
         if extrapolation is not None:
             from ..learned import run_extrapolated_als

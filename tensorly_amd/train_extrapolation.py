@@ -4,7 +4,7 @@ Data: the batched CP-ALS kernel in its probing policy (``tla_batched_cp_als``
 policy 3) runs ALS on synthetic problems; at each probed decision it records
 the 16 history features and, for every step ``alpha`` of a grid, the
 relative error after one sweep from ``A + alpha (A - A_prev)``.  The
-regression target is ``log2(1 + alpha*)`` of the best probed step.  Two
+regression target is ``log2(1 + alpha*)`` of the best probed step.  Three
 problem families are mixed:
 
 * the batched workload's distribution -- 3-way tensors with mode sizes
@@ -12,7 +12,10 @@ problem families are mixed:
   ("swamp" regime), 1 % noise;
 * dense CP models with U[0,1) factors (mode sizes U{32..64}, ranks
   U{8..16}, 1 % noise), the collinearity pattern of the 512^3 rank-32
-  workload at a size the batched kernel runs.
+  workload at a size the batched kernel runs;
+* larger U[0,1)-factor models (mode sizes U{96..256}, ranks U{16..32}),
+  probed with the single-problem driver along greedy trajectories, repeated
+  so that they make up about a fifth of the samples.
 
 Probing starts at a different decision per chunk, so the samples cover the
 start, the middle and the tail of the trajectories; decisions before the
@@ -53,6 +56,33 @@ def uniform01_problems(n, seed, size_range=(32, 64), rank_range=(8, 16), noise=0
         x = synthetic.add_noise(synthetic.cp_model(np.ones(r), fs), noise, rng).astype(np.float32)
         out.append((x, r))
     return out
+
+
+def large_uniform01_data(n, seed, device, size_range=(96, 256), rank_range=(16, 32), n_sweeps=64, log=print):
+    """Greedy probed trajectories (tensorly_amd.learned.generate_training_data)
+    of U[0,1)-factor CP models built on the GPU; rng(seed) draws sizes and
+    ranks, torch generator seed + i the factors and noise of problem i."""
+    from .learned import generate_training_data
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    Xs, ys = [], []
+    t0 = time.time()
+    for i in range(n):
+        shape = tuple(int(v) for v in rng.integers(size_range[0], size_range[1] + 1, size=3))
+        r = int(rng.integers(rank_range[0], rank_range[1] + 1))
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed + i)
+        fs = [torch.rand((s_, r), generator=g, device=dev, dtype=torch.float64) for s_ in shape]
+        x = torch.einsum("ir,jr,kr->ijk", *fs)
+        e = torch.randn(x.shape, generator=g, device=dev, dtype=torch.float64)
+        x = (x + 0.01 * torch.linalg.vector_norm(x) / torch.linalg.vector_norm(e) * e).float().contiguous()
+        X, y, _ = generate_training_data([(x, r)], n_sweeps=n_sweeps, warmup=3, alphas=PROBE_GRID, seed=seed + i)
+        Xs.append(X)
+        ys.append(y)
+        del x, e, fs
+        if (i + 1) % 25 == 0:
+            log(f"[train] large problems {i + 1}/{n} in {time.time() - t0:.1f}s")
+    return np.concatenate(Xs), np.concatenate(ys)
 
 
 def collect(problems, first_decision, max_decisions=6, alpha_fixed=2.0, seed=0, device=None, model=None):
@@ -125,6 +155,7 @@ def main(argv=None):
     p.add_argument("--epochs", type=int, default=30)
     p.add_argument("--seed", type=int, default=1000)
     p.add_argument("--out", default=default_model_path())
+    p.add_argument("--large", type=int, default=300, help="larger U[0,1) problems probed with the single driver")
     args = p.parse_args(argv)
 
     def log(msg):
@@ -132,7 +163,18 @@ def main(argv=None):
 
     t0 = time.time()
     X, y, meta = build_dataset(args.problems, seed=args.seed, log=log)
-    log(f"[train] {len(y)} samples in {time.time() - t0:.1f}s; mean alpha* {np.mean(2.0 ** y - 1):.3f}")
+    n_small = len(y)
+    n_large = 0
+    if args.large:
+        XL, yL = large_uniform01_data(args.large, args.seed + 777, None, log=log)
+        n_large = len(yL)
+        rep = max(1, int(round(0.25 * n_small / max(1, len(yL)))))
+        X = np.concatenate([X] + [XL] * rep)
+        y = np.concatenate([y] + [yL] * rep)
+        meta.append({"large_uniform01_problems": args.large, "samples": n_large, "repeated": rep,
+                     "mean_alpha_star": float(np.mean(2.0 ** yL - 1))})
+    log(f"[train] {len(y)} samples ({n_small} batched, {n_large} large) in {time.time() - t0:.1f}s; "
+        f"mean alpha* {np.mean(2.0 ** y - 1):.3f}")
     net, hist, base = train(X, y, epochs=args.epochs, log=log)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     info = {"problems": args.problems, "samples": int(len(y)), "probe_grid": list(PROBE_GRID), "chunks": meta,
