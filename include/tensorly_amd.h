@@ -264,7 +264,9 @@ int tla_extrap_features(const double* errors, const float* alphas, const int* ac
  *          1  the reference line search (_cp.py:333-336,401-403,448-483);
  *          2  learned (mlp_params) or fixed (alpha_fixed, mlp_params NULL)
  *             extrapolation, verified by the next sweep, after `warmup`
- *             plain sweeps;
+ *             plain sweeps; a rejected step pauses extrapolation for 1, 2,
+ *             4 or 8 plain sweeps (doubling while rejections follow each
+ *             other, reset by an accepted step);
  *          3  probing: as 2, but at decisions first_decision ..
  *             first_decision + max_decisions - 1 every step in
  *             probe_alphas[n_probe] is tried (one sweep each, errors to

@@ -532,6 +532,10 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
       }
       prev = e;
     }
+    // After a rejected step the next `skip` plain sweeps are not followed by
+    // an extrapolation; the pause doubles with consecutive rejections (up to
+    // 8) and resets on acceptance.  Not used when probing.
+    int backoff = 0, skip = 0;
     while (!done && !status && s < a.max_sweeps) {
       store_factors<NW>(p.prev);
       const double e = sweep<NW>(normx2);
@@ -543,6 +547,11 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
       if (s >= 1 && fabs(prev - e) < a.tol) break;
       prev = e;
       if (s + 1 >= a.max_sweeps) break;
+      if (skip > 0) {
+        --skip;
+        ++s;
+        continue;
+      }
       // the features read the history written by thread 0 (same thread)
       if (lead) extrap_features(herr, halpha, hacc, s, sm.feat);
       __syncthreads();
@@ -587,12 +596,17 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
         record(s, e2, alpha, 1);
         err = e2;
         prev = e2;
+        backoff = 0;
       } else {
         load_factors<NW>(p.save);
         grams_all<NW>();
         record(s, e, alpha, 0);
         err = e;
         prev = e;
+        if (!probe) {
+          backoff = backoff ? min(2 * backoff, 8) : 1;
+          skip = backoff;
+        }
       }
       ++s;
     }
