@@ -86,21 +86,25 @@ def measured_traffic():
 
 def raw_launcher(X, facs, prec, kind, arg, torch):
     """One C-ABI MTTKRP call with every buffer allocated up front, so the
-    timed region holds only the kernel launch(es)."""
+    timed region holds only the kernel launch(es).  X: float32 or bfloat16,
+    any order; facs: float32 factors (rank from their columns)."""
     from tensorly_amd import _lib
     L = _lib.lib()
     pcode = _lib.PRECISIONS[prec]
+    xcode = _lib.dtype_code(X)
+    nd = X.dim()
+    rank = int(facs[0].shape[1])
     sh, keep_sh = _lib.shape_array(X.shape)
     if kind == "partial":
         kb, ke = arg
         fl = [None if kb <= i < ke else f for i, f in enumerate(facs)]
         rows = int(np.prod(X.shape[kb:ke]))
-        out = torch.empty((rows, RANK), dtype=torch.float32, device=X.device)
-        nb = L.tla_mttkrp_partial_workspace_size(_lib.F32, 3, sh, RANK, kb, ke, pcode)
+        out = torch.empty((rows, rank), dtype=torch.float32, device=X.device)
+        nb = L.tla_mttkrp_partial_workspace_size(xcode, nd, sh, rank, kb, ke, pcode)
     else:
         fl = list(facs)
-        out = torch.empty((X.shape[arg], RANK), dtype=torch.float32, device=X.device)
-        nb = L.tla_mttkrp_workspace_size(_lib.F32, 3, sh, RANK, arg, pcode)
+        out = torch.empty((X.shape[arg], rank), dtype=torch.float32, device=X.device)
+        nb = L.tla_mttkrp_workspace_size(xcode, nd, sh, rank, arg, pcode)
     ws = torch.empty(max(int(nb), 1), dtype=torch.uint8, device=X.device)
     pp, keep_f = _lib.ptr_array(fl)
     st = _lib.stream(X.device)
@@ -108,9 +112,9 @@ def raw_launcher(X, facs, prec, kind, arg, torch):
 
     def fn():
         if kind == "partial":
-            rc = L.tla_mttkrp_partial(_lib.F32, xp, 3, sh, pp, None, RANK, arg[0], arg[1], pcode, op, wp, nb, st)
+            rc = L.tla_mttkrp_partial(xcode, xp, nd, sh, pp, None, rank, arg[0], arg[1], pcode, op, wp, nb, st)
         else:
-            rc = L.tla_mttkrp(_lib.F32, xp, 3, sh, pp, None, RANK, arg, pcode, op, wp, nb, st)
+            rc = L.tla_mttkrp(xcode, xp, nd, sh, pp, None, rank, arg, pcode, op, wp, nb, st)
         _lib.check(rc, kind)
     fn._keep = (keep_sh, keep_f, out, ws)
     return fn
@@ -188,8 +192,10 @@ def precision_study(torch, dev):
     """128^4 float32 (rank-64 CP model, unit N(0,1) columns, weights
     log-uniform in [1e-2, 1], 0.1 % noise): per-mode MTTKRP relative error
     against the FP64 numpy oracle for FP32, BF16 and bf16x3 (with and
-    without power-of-two column scaling), then the ALS fit after 200 sweeps
-    in each precision next to an FP64 GPU run."""
+    without power-of-two column scaling), per-MTTKRP kernel time; then 200
+    ALS sweeps in each precision from the reference's SVD init (next to an
+    FP64 GPU run) and from a random init, with the first non-finite sweep if
+    any."""
     import tensorly_amd as tla
     from oracle import tensorly_np as T
     from tensorly_amd import synthetic
@@ -230,35 +236,64 @@ def precision_study(torch, dev):
         out["mttkrp_rel_error"][f"mode{mode}"] = errs
         log(f"[study] mode {mode}: {errs}")
     del x64
-    fits = {}
-    ones = torch.ones(R, dtype=torch.float32, device=dev)
+    # per-MTTKRP kernel time (raw C-ABI launches, HIP events, 10 calls after
+    # one warm-up): 2 R N flop per MTTKRP; bytes = tensor + factors + output
+    n_el = int(np.prod(x.shape))
+    gflop = 2.0 * R * n_el / 1e9
+    out["mttkrp_gflop"] = gflop
+    out["mttkrp_time"] = {}
     for prec in ("fp32", "bf16", "bf16x3"):
-        eng = CpAlsEngine(X, ones, fg, precision=prec, dimtree=(prec != "fp32"),
+        src = Xb if prec == "bf16" else X
+        xbytes = n_el * (2 if prec == "bf16" else 4)
+        row = {}
+        for kind, arg, name in [("mode", m, f"mode{m}") for m in range(4)] + \
+                [("partial", (0, 2), "partial01"), ("partial", (2, 4), "partial23")]:
+            ms = kernel_time_ms(raw_launcher(src, fg, prec, kind, arg, torch), 10, torch)
+            row[name] = {"ms": round(ms, 4), "tflops": round(gflop / ms, 2),
+                         "gbs": round((xbytes + 4 * 128 * R * 4) / ms / 1e6, 1)}
+        out["mttkrp_time"][prec] = row
+        log(f"[study] MTTKRP time {prec}: {row}")
+    ones = torch.ones(R, dtype=torch.float32, device=dev)
+
+    def als(prec, f0, n, x=X, w=ones, normalize=False):
+        eng = CpAlsEngine(x, w, f0, precision=prec, dimtree=(prec in ("bf16", "bf16x3")),
                           tensor_bf16=(Xb if prec == "bf16" else None))
-        errs = torch.zeros(200, dtype=torch.float64, device=dev)
+        errs = torch.zeros(n, dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        eng.sweeps(200, errs, 0, False)
+        eng.sweeps(n, errs, 0, normalize)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
-        e = errs.cpu().numpy()
-        fits[prec] = {"final_error": float(e[-1]), "fit": float(1 - e[-1]), "ms_per_sweep": round(dt / 200 * 1e3, 4)}
         eng.close()
-        log(f"[study] ALS {prec}: {fits[prec]}")
+        e = errs.cpu().numpy()
+        bad = np.flatnonzero(~np.isfinite(e))
+        marks = [s for s in (0, 9, 24, 49, 99, 149, 199) if s < n]
+        return {"final_error": float(e[-1]), "fit": float(1 - e[-1]), "ms_per_sweep": round(dt / n * 1e3, 4),
+                "first_nonfinite_sweep": int(bad[0]) if len(bad) else None,
+                "error_at_sweep": {str(s + 1): float(e[s]) for s in marks}}
+
+    # (1) the reference's default init='svd' (computed once in FP64, cast):
+    # every precision starts from the same point
     X64 = X.double()
-    eng = CpAlsEngine(X64, ones.double(), [f.double() for f in fg], precision="valu")
-    errs = torch.zeros(200, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    eng.sweeps(200, errs, 0, False)
-    torch.cuda.synchronize()
-    e = errs.cpu().numpy()
-    fits["fp64"] = {"final_error": float(e[-1]), "fit": float(1 - e[-1]),
-                    "ms_per_sweep": round((time.perf_counter() - t1) / 200 * 1e3, 4)}
-    eng.close()
+    _, f_svd = tla.decomposition.initialize_cp(X64, R, init="svd")
+    fits = {}
     for prec in ("fp32", "bf16", "bf16x3"):
+        fits[prec] = als(prec, [f.float() for f in f_svd], 200)
+        log(f"[study] ALS svd init {prec}: {fits[prec]}")
+    # BF16 with the factors renormalised after every sweep (weights carry the scale)
+    fits["bf16_normalized"] = als("bf16", [f.float() for f in f_svd], 200, normalize=True)
+    log(f"[study] ALS svd init bf16 normalised: {fits['bf16_normalized']}")
+    fits["fp64"] = als("valu", f_svd, 200, x=X64, w=ones.double())
+    log(f"[study] ALS svd init fp64: {fits['fp64']}")
+    for prec in ("fp32", "bf16", "bf16x3", "bf16_normalized"):
         fits[prec]["fit_minus_fp64"] = fits[prec]["fit"] - fits["fp64"]["fit"]
-    out["als_200_sweeps"] = fits
+    out["als_svd_init_200_sweeps"] = fits
+    del X64
+    # (2) init='random' (RandomState(2)): reports where a run stops being finite
+    out["als_random_init_200_sweeps"] = {}
+    for prec in ("fp32", "bf16", "bf16x3"):
+        out["als_random_init_200_sweeps"][prec] = als(prec, fg, 200)
+        log(f"[study] ALS random init {prec}: {out['als_random_init_200_sweeps'][prec]}")
     return out
 
 

@@ -288,6 +288,12 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   TLA_REQUIRE(X && factors && workspace && handle_out, "cp_als: NULL pointer");
   if (x_dtype == TLA_F64) precision = TLA_PREC_VALU;
   TLA_REQUIRE(x_dtype != TLA_BF16 || precision == TLA_PREC_BF16, "cp_als: bf16 tensor needs bf16 precision");
+  // a bf16 copy has no generic (non-MFMA) MTTKRP: the inner kernels of modes
+  // 0..N-2 need the contiguous mode % 32 == 0, the last-mode kernel needs
+  // mode N-2 % 8 == 0
+  TLA_REQUIRE(x_dtype != TLA_BF16 || (shape[ndim - 1] % 32 == 0 && shape[ndim - 2] % 8 == 0),
+              "cp_als: bf16 tensor copy needs shape[%d] %% 32 == 0 and shape[%d] %% 8 == 0 (got %lld, %lld)",
+              ndim - 1, ndim - 2, (long long)shape[ndim - 1], (long long)shape[ndim - 2]);
   if (dimtree && !dimtree_ok(x_dtype, ndim, shape, rank, precision)) dimtree = 0;
   const Layout L = layout(x_dtype, ndim, shape, rank, precision, dimtree);
   TLA_REQUIRE(ws_bytes >= L.total, "cp_als: workspace too small (%zu < %zu)", ws_bytes, L.total);

@@ -9,12 +9,15 @@
 //  * n <= 64: cyclic Jacobi on the whole matrix in LDS (round-robin pair
 //    ordering, all n/2 rotations of a round applied in parallel).
 //  * n  > 64: block subspace iteration with Rayleigh-Ritz.  Block b = 64
-//    (k <= 56).  Each step: Z = G V (FP64), H = V^T Z, Jacobi(H) = W diag(t)
-//    W^T (warm: the previous Ritz basis nearly diagonalises H), residuals of
-//    the k leading Ritz pairs, V <- orth(Z W) by shifted Cholesky-QR (three
-//    passes).  Converged when every leading residual is below tol * t_1.
+//    for k <= 56 (Ritz matrix and rotations in LDS); otherwise b =
+//    min(n, 128) for k <= 120 or n <= 128 (Ritz matrix in LDS, rotations in
+//    the workspace; b = n makes the first Rayleigh-Ritz step exact).  Each
+//    step: Z = G V (FP64), H = V^T Z, Jacobi(H) = W diag(t) W^T (warm: the
+//    previous Ritz basis nearly diagonalises H), residuals of the k leading
+//    Ritz pairs, V <- orth(Z W) by shifted Cholesky-QR (three passes).
+//    Converged when every leading residual is below tol * t_1.
 // One workgroup per problem; the n x b blocks live in the caller's
-// workspace (L2-resident), the b x b matrices in LDS.
+// workspace (L2-resident).
 #include <math.h>
 
 #include <algorithm>
@@ -24,13 +27,16 @@
 namespace tla {
 
 constexpr int kEigThreads = 512;
-constexpr int kJac = 64;  // max Jacobi size (LDS)
+constexpr int kJac = 64;          // max Jacobi size (LDS)
+constexpr int kEigCapLarge = 128;  // larger subspace block
 
 // ---------------------------------------------------------------------------
-// Jacobi on an m x m symmetric matrix A (LDS, row-major, ld = kJac),
-// accumulating rotations into W (initialised by the caller).  Returns the
-// number of sweeps.
+// Jacobi on an m x m symmetric matrix A (row-major, leading dimension LD),
+// accumulating rotations into W (same layout, initialised by the caller).
+// A lives in LDS; W may be in LDS or in global memory (the LD = 128 case,
+// where A alone takes 128 KB of LDS).  Returns the number of sweeps.
 // ---------------------------------------------------------------------------
+template <int LD>
 __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int me = m + (m & 1);  // even number of slots (one dummy)
@@ -40,7 +46,7 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < m * m; e += nt) {
       const int i = e / m, j = e % m;
-      const double v = A[i * kJac + j];
+      const double v = A[i * LD + j];
       tot += v * v;
       if (i != j) off += v * v;
     }
@@ -66,9 +72,9 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         if (p > q) { const int t = p; p = q; q = t; }
         double c = 1.0, s = 0.0;
         if (q < m) {
-          const double apq = A[p * kJac + q];
+          const double apq = A[p * LD + q];
           if (fabs(apq) > 1e-300) {
-            const double app = A[p * kJac + p], aqq = A[q * kJac + q];
+            const double app = A[p * LD + p], aqq = A[q * LD + q];
             const double tau = (aqq - app) / (2.0 * apq);
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -88,9 +94,9 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         const int p = pr[2 * a], q = pr[2 * a + 1];
         if (q < m) {
           const double c = cs[2 * a], s = cs[2 * a + 1];
-          const double x = A[p * kJac + j], y = A[q * kJac + j];
-          A[p * kJac + j] = c * x - s * y;
-          A[q * kJac + j] = s * x + c * y;
+          const double x = A[p * LD + j], y = A[q * LD + j];
+          A[p * LD + j] = c * x - s * y;
+          A[q * LD + j] = s * x + c * y;
         }
       }
       __syncthreads();
@@ -100,13 +106,13 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         const int p = pr[2 * a], q = pr[2 * a + 1];
         if (q < m) {
           const double c = cs[2 * a], s = cs[2 * a + 1];
-          double x = A[i * kJac + p], y = A[i * kJac + q];
-          A[i * kJac + p] = c * x - s * y;
-          A[i * kJac + q] = s * x + c * y;
-          x = W[i * kJac + p];
-          y = W[i * kJac + q];
-          W[i * kJac + p] = c * x - s * y;
-          W[i * kJac + q] = s * x + c * y;
+          double x = A[i * LD + p], y = A[i * LD + q];
+          A[i * LD + p] = c * x - s * y;
+          A[i * LD + q] = s * x + c * y;
+          x = W[i * LD + p];
+          y = W[i * LD + q];
+          W[i * LD + p] = c * x - s * y;
+          W[i * LD + q] = s * x + c * y;
         }
       }
       __syncthreads();
@@ -116,13 +122,14 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
 }
 
 // order[r] = index of the r-th largest diagonal entry of A (ties by index)
+template <int LD>
 __device__ void sort_desc(const double* A, int m, int* order) {
   const int tid = threadIdx.x;
   if (tid < m) {
-    const double v = A[tid * kJac + tid];
+    const double v = A[tid * LD + tid];
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double u = A[j * kJac + j];
+      const double u = A[j * LD + j];
       if (u > v || (u == v && j < tid)) ++rank;
     }
     order[rank] = tid;
@@ -134,17 +141,19 @@ __device__ void sort_desc(const double* A, int m, int* order) {
 struct EigArgs {
   const double* G;
   int n, k;
-  const double* Q0;  // n x k0 warm start (row-major) or nullptr
-  int k0;
+  const double* Q0;  // warm start (row-major, ld q0_ld) or nullptr; its first k0 columns are used
+  int k0, q0_ld;
   double* evals;     // k
   double* evecs;     // n x k row-major
   double tol;
   int max_iter;
   int* iters;
-  double* V;  // n x 64
-  double* Z;  // n x 64
-  double* Y;  // n x 64
-  double* U;  // n x 64
+  int b;      // subspace block: columns in use, <= the kernel's capacity
+  double* V;  // n x capacity
+  double* Z;  // n x capacity
+  double* Y;  // n x capacity
+  double* U;  // n x capacity
+  double* W;  // capacity x capacity (Ritz rotations; used when capacity = 128)
 };
 
 __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
@@ -153,8 +162,9 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
   return (double)h * (1.0 / 4294967296.0) - 0.5;
 }
 
-// Shifted Cholesky-QR of the n x b block M (global, row-major ld 64) in
-// place; S (LDS, ld kJac) scratch.
+// Shifted Cholesky-QR of the n x b block M (global, row-major ld LD) in
+// place; S (LDS, ld LD) scratch.
+template <int LD>
 __device__ void block_orth(double* M, int n, int b, double* S, double* scratch, int* flag) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int pass = 0; pass < 3; ++pass) {
@@ -163,14 +173,14 @@ __device__ void block_orth(double* M, int n, int b, double* S, double* scratch, 
       const int i = e / b, j = e % b;
       if (j < i) continue;
       double s = 0.0;
-      for (int r = 0; r < n; ++r) s = fma(M[r * 64 + i], M[r * 64 + j], s);
-      S[i * kJac + j] = s;
-      S[j * kJac + i] = s;
+      for (int r = 0; r < n; ++r) s = fma(M[r * LD + i], M[r * LD + j], s);
+      S[i * LD + j] = s;
+      S[j * LD + i] = s;
     }
     __syncthreads();
     if (tid == 0) {
       double tr = 0.0;
-      for (int i = 0; i < b; ++i) tr = fmax(tr, S[i * kJac + i]);
+      for (int i = 0; i < b; ++i) tr = fmax(tr, S[i * LD + i]);
       scratch[40] = tr;
     }
     __syncthreads();
@@ -180,26 +190,26 @@ __device__ void block_orth(double* M, int n, int b, double* S, double* scratch, 
     __syncthreads();
     for (int j = 0; j < b; ++j) {
       if (tid == 0) {
-        double d = S[j * kJac + j] + shift;
-        for (int l = 0; l < j; ++l) d -= S[l * kJac + j] * S[l * kJac + j];
+        double d = S[j * LD + j] + shift;
+        for (int l = 0; l < j; ++l) d -= S[l * LD + j] * S[l * LD + j];
         if (!(d > 0.0)) { d = 1e-300; *flag = 1; }
-        S[j * kJac + j] = sqrt(d);
+        S[j * LD + j] = sqrt(d);
       }
       __syncthreads();
       for (int i = j + 1 + tid; i < b; i += nt) {
-        double v = S[j * kJac + i];
-        for (int l = 0; l < j; ++l) v -= S[l * kJac + j] * S[l * kJac + i];
-        S[j * kJac + i] = v / S[j * kJac + j];
+        double v = S[j * LD + i];
+        for (int l = 0; l < j; ++l) v -= S[l * LD + j] * S[l * LD + i];
+        S[j * LD + i] = v / S[j * LD + j];
       }
       __syncthreads();
     }
     // M = M R^-1 (row-wise forward substitution)
     for (int r = tid; r < n; r += nt) {
-      double* row = M + r * 64;
+      double* row = M + r * LD;
       for (int j = 0; j < b; ++j) {
         double v = row[j];
-        for (int l = 0; l < j; ++l) v -= row[l] * S[l * kJac + j];
-        row[j] = v / S[j * kJac + j];
+        for (int l = 0; l < j; ++l) v -= row[l] * S[l * LD + j];
+        row[j] = v / S[j * LD + j];
       }
     }
     __syncthreads();
@@ -220,8 +230,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigArgs a) {
     W[i * kJac + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int sweeps = lds_jacobi(A, W, n, 60, cs, pr, scratch);
-  sort_desc(A, n, order);
+  const int sweeps = lds_jacobi<kJac>(A, W, n, 60, cs, pr, scratch);
+  sort_desc<kJac>(A, n, order);
   for (int e = tid; e < n * a.k; e += blockDim.x) {
     const int i = e / a.k, r = e % a.k;
     a.evecs[i * a.k + r] = W[i * kJac + order[r]];
@@ -230,17 +240,24 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigArgs a) {
   if (tid == 0 && a.iters) *a.iters = sweeps;
 }
 
+// Block subspace iteration with Rayleigh-Ritz.  CAP is the block capacity:
+// 64 keeps H and the rotations W in LDS; 128 keeps H (128 KB) in LDS and W
+// in the workspace.  a.b <= CAP columns are used; with a.b == n the first
+// Rayleigh-Ritz step is already exact.
+template <int CAP>
 __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a) {
-  __shared__ double H[kJac * kJac];
-  __shared__ double W[kJac * kJac];
-  __shared__ double cs[kJac + 2];
-  __shared__ int pr[kJac + 2];
+  constexpr int LD = CAP;
+  __shared__ double H[LD * LD];
+  __shared__ double Wl[CAP == 64 ? LD * LD : 1];
+  __shared__ double cs[LD + 2];
+  __shared__ int pr[LD + 2];
   __shared__ double scratch[64];
-  __shared__ int order[kJac];
-  __shared__ double theta[kJac];
+  __shared__ double resid[LD];
+  __shared__ int order[LD];
+  __shared__ double theta[LD];
   __shared__ int flag;
-  const int n = a.n, k = a.k, tid = threadIdx.x, nt = blockDim.x;
-  const int b = 64;
+  const int n = a.n, k = a.k, b = a.b, tid = threadIdx.x, nt = blockDim.x;
+  double* W = (CAP == 64) ? Wl : a.W;
   double* V = a.V;
   double* Z = a.Z;
   double* Y = a.Y;
@@ -248,10 +265,10 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
   // V0: warm start columns, then deterministic pseudo-random columns
   for (int e = tid; e < n * b; e += nt) {
     const int i = e / b, j = e % b;
-    V[i * 64 + j] = (a.Q0 && j < a.k0) ? a.Q0[i * a.k0 + j] : hash_unit(i, j);
+    V[i * LD + j] = (a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j);
   }
   __syncthreads();
-  block_orth(V, n, b, H, scratch, &flag);
+  block_orth<LD>(V, n, b, H, scratch, &flag);
   int it = 0;
   for (; it < a.max_iter; ++it) {
     // Z = G V
@@ -259,8 +276,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
       const int i = e / b, j = e % b;
       const double* g = a.G + (int64_t)i * n;
       double s = 0.0;
-      for (int l = 0; l < n; ++l) s = fma(g[l], V[l * 64 + j], s);
-      Z[i * 64 + j] = s;
+      for (int l = 0; l < n; ++l) s = fma(g[l], V[l * LD + j], s);
+      Z[i * LD + j] = s;
     }
     __syncthreads();
     // H = V^T Z (symmetrised), W = I
@@ -269,16 +286,16 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
       if (j < i) continue;
       double s = 0.0, t = 0.0;
       for (int r = 0; r < n; ++r) {
-        s = fma(V[r * 64 + i], Z[r * 64 + j], s);
-        t = fma(V[r * 64 + j], Z[r * 64 + i], t);
+        s = fma(V[r * LD + i], Z[r * LD + j], s);
+        t = fma(V[r * LD + j], Z[r * LD + i], t);
       }
-      H[i * kJac + j] = H[j * kJac + i] = 0.5 * (s + t);
+      H[i * LD + j] = H[j * LD + i] = 0.5 * (s + t);
     }
-    for (int e = tid; e < b * b; e += nt) W[(e / b) * kJac + e % b] = ((e / b) == (e % b)) ? 1.0 : 0.0;
+    for (int e = tid; e < b * b; e += nt) W[(e / b) * LD + e % b] = ((e / b) == (e % b)) ? 1.0 : 0.0;
     __syncthreads();
-    lds_jacobi(H, W, b, 30, cs, pr, scratch);
-    sort_desc(H, b, order);
-    if (tid < b) theta[tid] = H[order[tid] * kJac + order[tid]];
+    lds_jacobi<LD>(H, W, b, 30, cs, pr, scratch);
+    sort_desc<LD>(H, b, order);
+    if (tid < b) theta[tid] = H[order[tid] * LD + order[tid]];
     __syncthreads();
     // Y = V W (Ritz vectors, sorted by Ritz value), U = Z W = G Y
     for (int e = tid; e < n * b; e += nt) {
@@ -286,27 +303,27 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
       const int oj = order[j];
       double y = 0.0, z = 0.0;
       for (int l = 0; l < b; ++l) {
-        const double w = W[l * kJac + oj];
-        y = fma(V[i * 64 + l], w, y);
-        z = fma(Z[i * 64 + l], w, z);
+        const double w = W[l * LD + oj];
+        y = fma(V[i * LD + l], w, y);
+        z = fma(Z[i * LD + l], w, z);
       }
-      Y[i * 64 + j] = y;
-      U[i * 64 + j] = z;
+      Y[i * LD + j] = y;
+      U[i * LD + j] = z;
     }
     __syncthreads();
     // residual norms of the leading k pairs
-    if (tid < k) {
+    for (int r = tid; r < k; r += nt) {
       double r2 = 0.0;
       for (int i = 0; i < n; ++i) {
-        const double d = U[i * 64 + tid] - theta[tid] * Y[i * 64 + tid];
+        const double d = U[i * LD + r] - theta[r] * Y[i * LD + r];
         r2 += d * d;
       }
-      scratch[tid] = sqrt(r2);
+      resid[r] = sqrt(r2);
     }
     __syncthreads();
     if (tid == 0) {
       double worst = 0.0;
-      for (int r = 0; r < k; ++r) worst = fmax(worst, scratch[r]);
+      for (int r = 0; r < k; ++r) worst = fmax(worst, resid[r]);
       scratch[63] = (worst <= a.tol * fmax(fabs(theta[0]), 1e-300)) ? 1.0 : 0.0;
     }
     __syncthreads();
@@ -314,15 +331,15 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     __syncthreads();
     if (done || it + 1 == a.max_iter) break;
     // V <- orth(Z W)
-    for (int e = tid; e < n * b; e += nt) V[(e / b) * 64 + e % b] = U[(e / b) * 64 + e % b];
+    for (int e = tid; e < n * b; e += nt) V[(e / b) * LD + e % b] = U[(e / b) * LD + e % b];
     __syncthreads();
-    block_orth(V, n, b, H, scratch, &flag);
+    block_orth<LD>(V, n, b, H, scratch, &flag);
   }
   for (int e = tid; e < n * k; e += nt) {
     const int i = e / k, r = e % k;
-    a.evecs[i * k + r] = Y[i * 64 + r];
+    a.evecs[i * k + r] = Y[i * LD + r];
   }
-  if (tid < k) a.evals[tid] = theta[tid];
+  for (int r = tid; r < k; r += nt) a.evals[r] = theta[r];
   if (tid == 0 && a.iters) *a.iters = it + 1;
 }
 
@@ -440,7 +457,8 @@ extern "C" int tla_mode_gram64(int dtype, const void* X, int ndim, const int64_t
 
 extern "C" size_t tla_sym_eig_workspace_size(int n) {
   if (n <= kJac) return 0;
-  return 4 * align256(size_t(n) * 64 * 8);
+  // four n x 128 blocks and the 128 x 128 rotations (sized for either block)
+  return 4 * align256(size_t(n) * kEigCapLarge * 8) + align256(size_t(kEigCapLarge) * kEigCapLarge * 8);
 }
 
 extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs,
@@ -448,10 +466,11 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
   TLA_REQUIRE(n >= 1, "sym_eig: n = %d", n);
   TLA_REQUIRE(k >= 1 && k <= n, "sym_eig: k = %d not in [1, n = %d]", k, n);
   TLA_REQUIRE(G && evals && evecs, "sym_eig: NULL pointer");
+  TLA_REQUIRE(!Q0 || k0 >= 1, "sym_eig: warm start with k0 = %d columns", k0);
   TLA_REQUIRE(ws_bytes >= tla_sym_eig_workspace_size(n), "sym_eig: workspace too small");
   hipStream_t st = (hipStream_t)stream;
   EigArgs a{};
-  a.G = G; a.n = n; a.k = k; a.Q0 = Q0; a.k0 = Q0 ? std::min(k0, 64) : 0;
+  a.G = G; a.n = n; a.k = k; a.Q0 = Q0; a.q0_ld = k0;
   a.evals = evals; a.evecs = evecs; a.tol = tol > 0 ? tol : 1e-11; a.max_iter = max_iter > 0 ? max_iter : 200;
   a.iters = iters_out;
   if (n <= kJac) {
@@ -459,16 +478,25 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
-  if (k > 56) {
-    set_error("sym_eig: k = %d > 56 leading eigenpairs of an n = %d matrix is not supported", k, n);
+  const bool small_block = k <= 56;
+  if (!small_block && !(k <= kEigCapLarge - 8 || n <= kEigCapLarge)) {
+    set_error("sym_eig: k = %d > %d leading eigenpairs of an n = %d > %d matrix is not supported", k,
+              kEigCapLarge - 8, n, kEigCapLarge);
     return TLA_ERR_UNSUPPORTED;
   }
-  const size_t blk = align256(size_t(n) * 64 * 8);
+  const int cap = small_block ? 64 : kEigCapLarge;
+  const size_t blk = align256(size_t(n) * cap * 8);
+  a.b = small_block ? 64 : std::min(n, kEigCapLarge);
+  a.k0 = Q0 ? std::min(k0, a.b) : 0;
   a.V = (double*)ws;
   a.Z = (double*)((char*)ws + blk);
   a.Y = (double*)((char*)ws + 2 * blk);
   a.U = (double*)((char*)ws + 3 * blk);
-  sym_eig_subspace_kernel<<<1, kEigThreads, 0, st>>>(a);
+  a.W = (double*)((char*)ws + 4 * blk);
+  if (small_block)
+    sym_eig_subspace_kernel<64><<<1, kEigThreads, 0, st>>>(a);
+  else
+    sym_eig_subspace_kernel<kEigCapLarge><<<1, kEigThreads, 0, st>>>(a);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

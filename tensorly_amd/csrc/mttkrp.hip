@@ -31,6 +31,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -926,6 +927,7 @@ struct LastStreamArgs {
   int wpr, rsplit;
   int64_t G;        // blocks per column group
   int64_t nvisits;  // ceil(P / (32 * rsplit))
+  int ldT;          // > 0: F_j staged in LDS transposed, [RT * 16][ldT]
 };
 
 // One visit's operands: 8 rows of this lane's column quad plus the Khatri-Rao
@@ -1003,7 +1005,61 @@ __device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4
   }
 }
 
-template <int PREC, typename XT, int RT>
+// LDS variant: the slot holds only the tensor rows and the Gm row; the
+// F_j rows are read from the transposed LDS copy at visit time, which keeps
+// two slots and 4 rank tiles of accumulators within the register file.
+template <int RT>
+struct LastSlotL {
+  floatx4 x[8];
+  float gm[RT];
+  int jj;
+};
+
+template <typename XT, int RT>
+__device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* X, int64_t col, bool colok,
+                                          int64_t rowb, int64_t pe, int c, LastSlotL<RT>& sl) {
+  const int R = args.R;
+  const int64_t K = args.K, J = args.J;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int64_t row = rowb + u;
+    sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (row < pe && colok) sl.x[u] = load_x4<XT>(X + row * K + col);
+  }
+  // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0; a row
+  // block past the end reads F_j row 0 (its x and gm are zero)
+  const bool okrow = rowb < pe;
+  const int64_t aa = okrow ? rowb / J : 0;
+  sl.jj = okrow ? (int)(rowb - aa * J) : 0;
+#pragma unroll
+  for (int mb = 0; mb < RT; ++mb) {
+    const int r = mb * 16 + c;
+    sl.gm[mb] = (okrow && r < R) ? args.Gm[aa * R + r] : 0.0f;
+  }
+}
+
+template <int PREC, int RT>
+__device__ __forceinline__ void last_visit(const LastSlotL<RT>& sl, const float* fjs, int ldT, int c,
+                                           floatx4 acc[4][RT]) {
+  LastSlot<RT> t;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) t.x[u] = sl.x[u];
+#pragma unroll
+  for (int mb = 0; mb < RT; ++mb) {
+    t.gm[mb] = sl.gm[mb];
+    const float* f = fjs + (mb * 16 + c) * ldT + sl.jj;
+    const floatx4 f0 = *(const floatx4*)f;
+    const floatx4 f1 = *(const floatx4*)(f + 4);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      t.fj[u][mb] = f0[u];
+      t.fj[u + 4][mb] = f1[u];
+    }
+  }
+  last_visit<PREC, RT>(t, acc);
+}
+
+template <int PREC, typename XT, int RT, bool FJL>
 __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs args) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, g = lane >> 4;
@@ -1011,6 +1067,17 @@ __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs 
   const int64_t K = args.K;
   const int64_t cgroup = blockIdx.x / args.G;
   const int64_t b = blockIdx.x - cgroup * args.G;
+  extern __shared__ float fjs[];  // FJL: F_j transposed, [RT * 16][ldT]
+  if constexpr (FJL) {
+    const int64_t J = args.J;
+    const int ldT = args.ldT;
+    for (int64_t e = tid; e < J * RT * 16; e += 512) {
+      const int64_t j = e / (RT * 16);
+      const int r = (int)(e - j * (RT * 16));
+      fjs[r * ldT + j] = (r < R) ? args.Fj[j * R + r] : 0.0f;
+    }
+    __syncthreads();
+  }
   const int wc = wave % args.wpr, rs = wave / args.wpr;
   if (rs >= args.rsplit) return;  // only when wpr * rsplit < 8; no block-wide sync follows
   const int64_t ctile = cgroup * 8 + wc;
@@ -1026,17 +1093,21 @@ __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs 
   // visit v covers rows v * V + 32 * rs .. +31; lane rows p0 + 8 g + u.
   auto rowb_of = [&](int64_t v) { return v * V + 32 * rs + 8 * g; };
   auto pe_of = [&](int64_t v) { return min(args.P, v * V + 32 * rs + 32); };
-  LastSlot<RT> sa, sb;
+  using Slot = typename std::conditional<FJL, LastSlotL<RT>, LastSlot<RT>>::type;
+  const int ldT = args.ldT;
+  Slot sa, sb;
   int64_t va = b;
   if (va < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(va), pe_of(va), c, sa);
   for (; va < args.nvisits; va += 2 * args.G) {
     const int64_t vb = va + args.G;
     if (vb < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vb), pe_of(vb), c, sb);
-    last_visit<PREC, RT>(sa, acc);
+    if constexpr (FJL) last_visit<PREC, RT>(sa, fjs, ldT, c, acc);
+    else last_visit<PREC, RT>(sa, acc);
     if (vb >= args.nvisits) break;
     const int64_t vc = vb + args.G;
     if (vc < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vc), pe_of(vc), c, sa);
-    last_visit<PREC, RT>(sb, acc);
+    if constexpr (FJL) last_visit<PREC, RT>(sb, fjs, ldT, c, acc);
+    else last_visit<PREC, RT>(sb, acc);
   }
   // acc[t][mb][e] = out[col + t][mb * 16 + 4 g + e]
   const int64_t slot = b * args.rsplit + rs;
@@ -1340,6 +1411,13 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     int64_t G = std::max<int64_t>(1, target / cgroups);
     G = std::min<int64_t>(G, s.nvisits);
     s.G = G;
+    // F_j in LDS (transposed, row stride J + 4: 16-byte aligned rows that
+    // start 4 banks apart) when it fits; otherwise F_j rows ride in the slot
+    const size_t fj_lds = size_t(RT) * 16 * (J + 4) * 4;
+    const int fj_mode = getenv("TLA_LAST_FJ") ? atoi(getenv("TLA_LAST_FJ")) : -1;
+    const bool use_lds = fj_mode == 1 || (fj_mode != 0 && RT >= 3);
+    s.ldT = (use_lds && fj_lds <= 96 * 1024) ? (int)(J + 4) : 0;
+    p->smem = s.ldT ? fj_lds : 0;
     p->grid = dim3((unsigned)(G * cgroups), 1, 1);
     p->splits = (int)(G * s.rsplit);
     p->partial_elems = size_t(p->splits) * Kc * R;
@@ -1444,20 +1522,32 @@ static int launch_stream(const Plan& p, hipStream_t st) {
   return launch_stream_epi<PREC, XT, SE_ACCJ>(p, st);
 }
 
+template <int PREC, typename XT, int RT>
+static int launch_last_stream_rt(const Plan& p, hipStream_t st) {
+  const LastStreamArgs a = p.ls;
+  if (a.ldT > 0) {
+    auto* fn = &mttkrp_last_stream_kernel<PREC, XT, RT, true>;
+    const int rc = allow_lds(fn, p.smem);
+    if (rc) return rc;
+    fn<<<p.grid, 512, p.smem, st>>>(a);
+  } else {
+    mttkrp_last_stream_kernel<PREC, XT, RT, false><<<p.grid, 512, 0, st>>>(a);
+  }
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
 template <int PREC, typename XT>
 static int launch_last_stream(const Plan& p, hipStream_t st) {
-  LastStreamArgs a = p.ls;
   switch (p.RT) {
-    case 1: mttkrp_last_stream_kernel<PREC, XT, 1><<<p.grid, 512, 0, st>>>(a); break;
-    case 2: mttkrp_last_stream_kernel<PREC, XT, 2><<<p.grid, 512, 0, st>>>(a); break;
-    case 3: mttkrp_last_stream_kernel<PREC, XT, 3><<<p.grid, 512, 0, st>>>(a); break;
-    case 4: mttkrp_last_stream_kernel<PREC, XT, 4><<<p.grid, 512, 0, st>>>(a); break;
+    case 1: return launch_last_stream_rt<PREC, XT, 1>(p, st);
+    case 2: return launch_last_stream_rt<PREC, XT, 2>(p, st);
+    case 3: return launch_last_stream_rt<PREC, XT, 3>(p, st);
+    case 4: return launch_last_stream_rt<PREC, XT, 4>(p, st);
     default:
       set_error("rank tile %d unsupported", p.RT);
       return TLA_ERR_UNSUPPORTED;
   }
-  TLA_LAUNCH_CHECK();
-  return TLA_OK;
 }
 
 template <int PREC, typename XT>

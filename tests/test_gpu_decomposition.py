@@ -63,6 +63,44 @@ def test_parafac_order4_dimtree(cuda):
         assert np.max(np.abs(np.array(errs) - np.array(ref_errs))) < 1e-5
 
 
+@pytest.mark.parametrize("shape", [(48, 40, 64), (16, 24, 16, 32)])
+@pytest.mark.parametrize("dimtree", [False, True])
+def test_parafac_bf16_copy_tracks_fp64(cuda, shape, dimtree):
+    """ALS whose MTTKRPs read a bfloat16 copy of the tensor (BF16 MFMA, FP32
+    accumulation; ||X||^2 from the float32 data).  The factors it returns
+    reconstruct X to within 1e-5 (absolute, relative error) of the FP64
+    oracle's final error.  The reported error is the cheap form
+    sqrt(||X||^2 + ||M||^2 - 2 <X, M>) with <X, M> from the BF16 MTTKRP,
+    whose relative error (~1e-4 here) is amplified by the cancellation, so it
+    is checked on err^2 with a 1e-3 tolerance."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(11 + len(shape))
+    f_true = [rng.standard_normal((s, 5)) for s in shape]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(5), f_true, dtype=np.float32), 0.01, rng)
+    (_, _), ref_errs = T.parafac(x.astype(np.float64), 5, n_iter_max=30, init="random", random_state=3, tol=0,
+                                 return_errors=True)
+    xg = g(x, cuda, torch.float32)
+    cp, errs = tla.parafac(xg, 5, n_iter_max=30, init="random", random_state=3, tol=0, return_errors=True,
+                           precision="bf16", tensor_bf16=xg.to(torch.bfloat16), dimtree=dimtree)
+    errs = np.array(errs)
+    assert np.all(np.isfinite(errs))
+    x64 = x.astype(np.float64)
+    rec = T.cp_to_tensor((cp.weights.double().cpu().numpy(), [f.double().cpu().numpy() for f in cp.factors]))
+    true_err = np.linalg.norm(x64 - rec) / np.linalg.norm(x64)
+    assert abs(true_err - ref_errs[-1]) < 1e-5, (true_err, ref_errs[-1])
+    assert abs(errs[-1] ** 2 - ref_errs[-1] ** 2) < 1e-3, (errs[-1], ref_errs[-1])
+
+
+def test_parafac_bf16_copy_rejects_unsupported_shape(cuda):
+    """A bfloat16 copy has no generic MTTKRP: shapes outside the MFMA plans
+    (last mode not a multiple of 32) are rejected when the driver is made."""
+    import tensorly_amd as tla
+    xg = torch.rand((16, 24, 56), device=cuda)
+    with pytest.raises(ValueError, match="bf16 tensor copy"):
+        tla.parafac(xg, 4, n_iter_max=2, init="random", random_state=0, precision="bf16",
+                    tensor_bf16=xg.to(torch.bfloat16))
+
+
 def test_parafac_linesearch_matches_reference(cuda, trajectories):
     import tensorly_amd as tla
     x, _ = synthetic.cp_64_rank8()
@@ -107,7 +145,8 @@ def test_parafac_fixed_modes_l2_callback(cuda):
     assert len(seen) == 4
 
 
-@pytest.mark.parametrize("n,k", [(40, 5), (64, 64), (200, 32), (256, 56)])
+@pytest.mark.parametrize("n,k", [(40, 5), (64, 64), (200, 32), (256, 56), (100, 57), (128, 64), (128, 128),
+                                 (200, 100), (300, 120)])
 def test_gram_eigensolver(cuda, n, k):
     from tensorly_amd.svd import gram_eigh
     rng = np.random.default_rng(n)
@@ -121,6 +160,26 @@ def test_gram_eigensolver(cuda, n, k):
     np.testing.assert_allclose(evals.cpu().numpy(), ref_vals, rtol=1e-10)
     overlap = np.abs(np.sum(evecs.cpu().numpy() * ref_vecs, axis=0))
     np.testing.assert_allclose(overlap, np.ones(k), atol=1e-8)
+
+
+def test_gram_eigensolver_rejects_large_k(cuda):
+    from tensorly_amd.svd import gram_eigh
+    G = torch.eye(200, dtype=torch.float64, device=cuda)
+    with pytest.raises(NotImplementedError, match="not supported"):
+        gram_eigh(G, 121)
+
+
+def test_svd_init_rank_above_56(cuda):
+    """init='svd' at rank 64 on 128-wide modes (the 64-column block would
+    hold at most 56 converged pairs) against the oracle's SVD init; the
+    24-wide mode is padded with the same RandomState draws."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((128, 96, 24))
+    _, f = tla.decomposition.initialize_cp(g(x, cuda), 64, init="svd", random_state=0)
+    _, f_ref = T.initialize_cp(x, 64, init="svd", random_state=0)
+    for m in range(3):
+        np.testing.assert_allclose(f[m].cpu().numpy(), f_ref[m], atol=1e-8)
 
 
 def test_svd_interface_sign_rule(cuda):

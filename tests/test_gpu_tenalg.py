@@ -106,19 +106,66 @@ def test_mttkrp_signed_data_bf16x3(cuda):
                 assert err < tol, (prec, mode, err)
 
 
-def test_mttkrp_bf16_copy(cuda):
+@pytest.mark.parametrize("shape", [(16, 32, 64), (8, 16, 24, 64)])
+@pytest.mark.parametrize("rank", [16, 48, 64])
+def test_mttkrp_bf16_copy(cuda, shape, rank):
     """A bfloat16 tensor is the BF16 copy of float32 data (MFMA path)."""
     import tensorly_amd as tla
-    rng = np.random.default_rng(6)
-    shape, rank = (16, 32, 64), 16
+    rng = np.random.default_rng(6 + rank)
     X = rng.random(shape).astype(np.float32)
     f = [rng.random((s, rank)).astype(np.float32) for s in shape]
     Xb = g(X, cuda, torch.float32).to(torch.bfloat16)
     fg = [g(x, cuda, torch.float32) for x in f]
-    for mode in range(3):
+    for mode in range(len(shape)):
         ref = T.unfolding_dot_khatri_rao(X.astype(np.float64), (None, [x.astype(np.float64) for x in f]), mode)
         err = rel(tla.unfolding_dot_khatri_rao(Xb, (None, fg), mode).cpu(), ref)
         assert err < 1e-2, (mode, err)
+
+
+@pytest.mark.parametrize("fj", ["0", "1"])
+@pytest.mark.parametrize("rank", [5, 16, 32, 64])
+def test_last_mode_fj_staging(cuda, monkeypatch, fj, rank):
+    """Last-mode kernel with the mode N-2 factor rows carried in registers
+    (TLA_LAST_FJ=0) or staged in LDS (TLA_LAST_FJ=1), every rank tile count,
+    against the FP64 oracle; both variants give the same bits."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(rank)
+    shape = (24, 40, 200)
+    X = rng.standard_normal(shape).astype(np.float32)
+    f = [rng.standard_normal((s, rank)).astype(np.float32) for s in shape]
+    Xg, fg = g(X, cuda, torch.float32), [g(x, cuda, torch.float32) for x in f]
+    ref = T.unfolding_dot_khatri_rao(X.astype(np.float64), (None, [x.astype(np.float64) for x in f]), 2)
+    for prec, tol in (("fp32", 1e-5), ("bf16x3", 3e-5), ("bf16", 3e-2)):
+        with tla.precision(prec):
+            monkeypatch.setenv("TLA_LAST_FJ", fj)
+            out = tla.unfolding_dot_khatri_rao(Xg, (None, fg), 2)
+            monkeypatch.setenv("TLA_LAST_FJ", "1" if fj == "0" else "0")
+            other = tla.unfolding_dot_khatri_rao(Xg, (None, fg), 2)
+        assert rel(out.cpu(), ref) < tol, (prec, rel(out.cpu(), ref))
+        assert torch.equal(out, other), prec
+
+
+@pytest.mark.parametrize("shape", [(32, 48, 64), (16, 24, 16, 64)])
+def test_partial_mttkrp_bf16_copy(cuda, shape):
+    """Dimension-tree partial contractions of a bfloat16 tensor (the BF16
+    copy path, orders 3 and 4) against the FP64 einsum of the float32 data."""
+    from tensorly_amd.tenalg import mttkrp_partial
+    rng = np.random.default_rng(60 + len(shape))
+    R = 16
+    X = rng.standard_normal(shape).astype(np.float32)
+    f = [rng.standard_normal((s, R)).astype(np.float32) for s in shape]
+    Xb = g(X, cuda, torch.float32).to(torch.bfloat16)
+    fg = [g(x, cuda, torch.float32) for x in f]
+    X64, f64 = X.astype(np.float64), [x.astype(np.float64) for x in f]
+    if len(shape) == 3:
+        cases = {(0, 2): np.einsum("abc,cr->abr", X64, f64[2])}
+    else:
+        cases = {(0, 2): np.einsum("abcd,cr,dr->abr", X64, f64[2], f64[3]),
+                 (2, 4): np.einsum("abcd,ar,br->cdr", X64, f64[0], f64[1])}
+    for keep, ref in cases.items():
+        got = mttkrp_partial(Xb, (None, fg), keep).cpu().numpy().reshape(ref.shape)
+        assert np.all(np.isfinite(got))
+        assert rel(got, ref) < 1e-2, (keep, rel(got, ref))
 
 
 def test_mttkrp_deterministic(cuda):
