@@ -8,6 +8,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "inverse.h"
 
 namespace tla {
 
@@ -87,9 +88,10 @@ __global__ __launch_bounds__(256) void gram_finalize_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Normal-equations solve.  Every block rebuilds V and inverts it in
-// registers (R <= 64: a few microseconds), then handles kSolveRows rows of
-// the new factor, its partial Gram and partial <M, A>.
+// Normal-equations solve.  Each block handles kSolveRows rows of the new
+// factor, its partial Gram and partial <M, A>, with V^-1 either computed
+// ahead (by the previous solve's finalising block, see next_mode) or rebuilt
+// and inverted by every block in registers.
 // ---------------------------------------------------------------------------
 struct SolveArgs {
   const double* grams[kMaxOrder];
@@ -110,281 +112,101 @@ struct SolveArgs {
   const double* normx2;
   double* errors;
   int err_slot;
-  // optional: V^-1 computed ahead (R x R FP64, global) and its status
+  // optional: V^-1 computed ahead (R x R FP64, global) and its status; a
+  // negative status means "not valid any more" (the Grams changed since):
+  // the blocks then invert V themselves
   const double* vi_pre;
   const int* vi_status;
+  // optional (needs counter): after finalising, the last block also forms
+  // and inverts V of mode next_mode from the Grams including this update's,
+  // into vi_next / vist_next, for the next solve (-1: none)
+  int next_mode;
+  double* vi_next;
+  int* vist_next;
 };
 
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// D = A B + C, A 16 x 4, B 4 x 16 (FP64 MFMA): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds D[l / 16 + 4 e][l % 16], e < 4 (rows
+// interleaved by 4, unlike the FP32 16x16x4 form).
+__device__ __forceinline__ doublex4 mfma_f64(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-
-// V^-1 of the symmetric positive definite V (R x R in LDS, left intact) by
-// Gauss-Jordan without pivoting,
-// held in registers: wave w owns rows w, w + 4, ... of [V | I], lane l owns
-// columns l and l + 64.  Per elimination step the owner of the pivot row
-// publishes it through LDS (double-buffered, one barrier per step) and every
-// row reads its multiplier from the pivot column with a lane read.  For SPD V
-// the pivots are the (positive) Schur complement diagonals; a pivot that is
-// not positive and finite returns false (the caller falls back to partial
-// pivoting).  Needs blockDim.x == 256 and R <= 64.
-__device__ __forceinline__ bool spd_gj_inverse(const double* V, int R, double* Vi, double* pbuf, int* flag) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nq = (R - w + 3) >> 2;
-  double x[16][2];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int i = w + 4 * q;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = lane + 64 * h;
-      double v = 0.0;
-      if (q < nq) v = (c < R) ? V[i * R + c] : ((c - R == i) ? 1.0 : 0.0);
-      x[q][h] = v;
-    }
-  }
-  for (int k = 0; k < R; ++k) {
-    const int qk = k >> 2, kl = k & 63, kh = k >> 6;
-    double* buf = pbuf + (k & 1) * 128;
-    if (w == (k & 3)) {
-      double r0 = 0.0, r1 = 0.0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q == qk) {
-          r0 = x[q][0];
-          r1 = x[q][1];
-        }
-      const double piv = readlane_f64(kh ? r1 : r0, kl);
-      const bool bad = !(piv > 0.0) || !isfinite(piv);
-      const double inv = 1.0 / piv;
-      r0 *= inv;
-      r1 *= inv;
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q == qk) {
-          x[q][0] = r0;
-          x[q][1] = r1;
-        }
-      buf[lane] = r0;
-      buf[64 + lane] = r1;
-      if (lane == 0) flag[k & 1] = bad ? 1 : 0;
-    }
-    __syncthreads();
-    if (flag[k & 1]) return false;
-    const double p0 = buf[lane], p1 = buf[64 + lane];
-    // branch-free over all 16 row slots: rows outside [0, R) hold zeros, the
-    // pivot row gets a zero multiplier (uniform selects, no divergent jumps)
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const double fr = readlane_f64(kh ? x[q][1] : x[q][0], kl);
-      const double f = (w + 4 * q != k) ? fr : 0.0;
-      x[q][0] = fma(-f, p0, x[q][0]);
-      x[q][1] = fma(-f, p1, x[q][1]);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int i = w + 4 * q;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = lane + 64 * h;
-      if (q < nq && c >= R && c < 2 * R) Vi[i * R + (c - R)] = x[q][h];
-    }
-  }
-  __syncthreads();
-  return true;
-}
-
-// Same elimination for R <= 32 inside ONE wave (lane c owns column c of
-// [V | I], rows in registers): no barriers and no LDS traffic per pivot, which
-// makes the R sequential steps cheap.  Called by wave 0 only; Vi and the
-// return value are valid after the caller's next barrier / in wave 0.
-__device__ __forceinline__ bool wave_spd_inverse32(const double* V, int R, double* Vi) {
-  const int lane = threadIdx.x & 63;
-  double x[32];
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    double v = 0.0;
-    if (i < R) v = (lane < R) ? V[i * R + lane] : ((lane - R == i) ? 1.0 : 0.0);
-    x[i] = v;
-  }
-  bool ok = true;
-#pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    if (k < R) {
-      const double piv = readlane_f64(x[k], k);
-      ok = ok && (piv > 0.0) && isfinite(piv);
-      x[k] *= 1.0 / piv;
-#pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        if (i != k) {
-          const double f = readlane_f64(x[i], k);
-          x[i] = fma(-f, x[k], x[i]);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 32; ++i)
-    if (i < R && lane >= R && lane < 2 * R) Vi[i * R + (lane - R)] = x[i];
-  return ok;
-}
-
-// Gauss-Jordan inverse with partial pivoting (fallback for V that is not
-// numerically positive definite).  A is destroyed.  Returns false if singular.
-__device__ bool lds_gj_inverse(double* A, double* Vi, int R, int* piv_shared, int* flag) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < R * R; e += blockDim.x) Vi[e] = ((e / R) == (e % R)) ? 1.0 : 0.0;
-  if (tid == 0) *flag = 0;
-  __syncthreads();
-  for (int k = 0; k < R; ++k) {
-    if (tid == 0) {
-      int p = k;
-      double best = fabs(A[k * R + k]);
-      for (int i = k + 1; i < R; ++i) {
-        const double v = fabs(A[i * R + k]);
-        if (v > best) { best = v; p = i; }
-      }
-      *piv_shared = p;
-      if (!(best > 0.0) || !isfinite(best)) *flag = 1;
-    }
-    __syncthreads();
-    if (*flag) return false;
-    const int p = *piv_shared;
-    if (p != k) {
-      for (int j = tid; j < R; j += blockDim.x) {
-        double t = A[k * R + j]; A[k * R + j] = A[p * R + j]; A[p * R + j] = t;
-        t = Vi[k * R + j]; Vi[k * R + j] = Vi[p * R + j]; Vi[p * R + j] = t;
-      }
-    }
-    __syncthreads();
-    const double inv = 1.0 / A[k * R + k];
-    __syncthreads();
-    for (int j = tid; j < R; j += blockDim.x) { A[k * R + j] *= inv; Vi[k * R + j] *= inv; }
-    __syncthreads();
-    for (int e = tid; e < R * R; e += blockDim.x) {
-      const int i = e / R, j = e % R;
-      if (i != k) {
-        const double f = A[i * R + k];
-        if (j != k) A[i * R + j] -= f * A[k * R + j];
-        Vi[i * R + j] -= f * Vi[k * R + j];
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < R; i += blockDim.x)
-      if (i != k) A[i * R + k] = 0.0;
-    __syncthreads();
-  }
-  return true;
-}
-
-template <typename T>
-__device__ void build_v(const SolveArgs& a, double* V) {
-  const int R = a.R;
-  for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
-    const int r = e / R, s = e % R;
-    double v = 1.0;
-    for (int m = 0; m < a.ndim; ++m)
-      if (m != a.mode) v *= a.grams[m][e];
-    if (r == s) v += a.l2;
-    if (a.weights) v *= ld<T>(a.weights, r) * ld<T>(a.weights, s);
-    V[e] = v;
-  }
-  __syncthreads();
-}
-
-// V (built from the Grams into Vbuf) and V^-1 into Vi, both LDS R x R.
-// Returns 0 (SPD elimination), 1 (partial-pivoting fallback) or 2 (singular).
-template <typename T>
-__device__ int invert_v(const SolveArgs& a, double* Vbuf, double* Vi, double* pbuf, int* pflag, int* piv,
-                        int* flag) {
-  const int R = a.R;
-  build_v<T>(a, Vbuf);  // V from the Grams, all threads (global loads in parallel)
-  bool spd_ok;
-  if (R <= 32) {
-    if (threadIdx.x < 64) {
-      const bool ok = wave_spd_inverse32(Vbuf, R, Vi);
-      if (threadIdx.x == 0) pflag[0] = ok ? 1 : 0;
-    }
-    __syncthreads();
-    spd_ok = pflag[0] != 0;
-  } else {
-    spd_ok = spd_gj_inverse(Vbuf, R, Vi, pbuf, pflag);
-  }
-  if (spd_ok) return 0;
-  return lds_gj_inverse(Vbuf, Vi, R, piv, flag) ? 1 : 2;
-}
-
-// V^-1 alone (one block), so that it can run on a second stream while the
-// MTTKRP of the same mode is still streaming the tensor.
-template <typename T>
-__global__ __launch_bounds__(256) void cp_inverse_kernel(SolveArgs a, double* vi_out, int* status_out) {
-  __shared__ double Vbuf[kMaxRank * kMaxRank];
-  __shared__ double Vi[kMaxRank * kMaxRank];
-  __shared__ double pbuf[256];
-  __shared__ int flag, piv, pflag[2];
-  const int st = invert_v<T>(a, Vbuf, Vi, pbuf, pflag, &piv, &flag);
-  __syncthreads();
-  for (int e = threadIdx.x; e < a.R * a.R; e += blockDim.x) vi_out[e] = Vi[e];
-  if (threadIdx.x == 0) *status_out = st;
-}
+static_assert(kSolveRows == 64, "the solve's MFMA tiling assigns 16 rows to each of 4 waves");
 
 template <typename T>
 __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
-  __shared__ double Lbuf[kMaxRank * kMaxRank];
-  __shared__ double Vi[kMaxRank * kMaxRank];
-  __shared__ double Mr[kSolveRows * kMaxRank];
-  __shared__ double pbuf[256];
+  __shared__ double Lbuf[kMaxRank * kMaxRank];  // V, then the rounded new rows (kSolveRows x RP)
+  __shared__ double Vi[kMaxRank * kMaxRank];    // V^-1, RP x RP
+  __shared__ double Mr[kSolveRows * kMaxRank];  // this block's MTTKRP rows, kSolveRows x RP, zero padded
+  __shared__ double pbuf[6 * kMaxRank];
   __shared__ int flag, piv, ticket, pflag[2];
-  const int R = a.R, tid = threadIdx.x;
+  const int R = a.R, RP = solve_pad(R), RT = RP / 16, tid = threadIdx.x;
+  constexpr int wdt = sizeof(T) == 8 ? TLA_F64 : TLA_F32;
+  const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
   const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
   const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
-  // this block's MTTKRP rows, staged once (coalesced)
-  for (int e = tid; e < nr * R; e += blockDim.x) Mr[e] = ld<T>(a.mttkrp, r0 * R + e);
-  int status = 0;
-  if (a.vi_pre) {
-    for (int e = tid; e < R * R; e += blockDim.x) Vi[e] = a.vi_pre[e];
-    status = *a.vi_status;
-    __syncthreads();
+  for (int e = tid; e < kSolveRows * RP; e += blockDim.x) {
+    const int i = e / RP, c = e % RP;
+    Mr[e] = (i < nr && c < R) ? ld<T>(a.mttkrp, (r0 + i) * R + c) : 0.0;
+  }
+  int status = a.vi_pre ? *a.vi_status : -1;  // the same word for every thread
+  if (status >= 0) {
+    for (int e = tid; e < RP * RP; e += blockDim.x) {
+      const int r = e / RP, s = e % RP;
+      Vi[e] = (r < R && s < R) ? a.vi_pre[r * R + s] : 0.0;
+    }
   } else {
-    status = invert_v<T>(a, Lbuf, Vi, pbuf, pflag, &piv, &flag);
+    status = invert_v<256>(a, a.ndim, a.mode, -1, nullptr, a.weights, wdt, a.l2, R, Lbuf, Vi, pbuf, pflag, &piv, &flag);
   }
   if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
   __syncthreads();
-  // New rows: A[i, s] = sum_r M[i, r] Vi[r, s]  (V symmetric); keep the
-  // rounded values in LDS (reusing Lbuf) for the Gram / inner product.
-  double* An = Lbuf;  // kSolveRows x R
+  // New rows A = M V^-1 (V symmetric) on FP64 MFMA: wave w computes rows
+  // 16 w .. 16 w + 15, every 16-column tile.  The rounded values go to the
+  // factor, to LDS (reusing Lbuf) for the Gram, and into <M, A>.
+  double* An = Lbuf;
   double ip = 0.0;
-  for (int e = tid; e < nr * R; e += blockDim.x) {
-    const int i = e / R, s = e % R;
-    const double* mrow = Mr + i * R;
-    double v0 = 0.0, v1 = 0.0;
-    int r = 0;
-    for (; r + 1 < R; r += 2) {
-      v0 = fma(mrow[r], Vi[r * R + s], v0);
-      v1 = fma(mrow[r + 1], Vi[(r + 1) * R + s], v1);
+  {
+    doublex4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < RP; k0 += 4) {
+      const double av = Mr[(16 * w + lr) * RP + k0 + lk];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t < RT) acc[t] = mfma_f64(av, Vi[(k0 + lk) * RP + 16 * t + lr], acc[t]);
     }
-    if (r < R) v0 = fma(mrow[r], Vi[r * R + s], v0);
-    const T out = (T)(v0 + v1);
-    ((T*)a.factor)[(r0 + i) * R + s] = out;
-    An[i * R + s] = (double)out;
-    ip = fma(mrow[s], (double)out, ip);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < RT) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 16 * w + lk + 4 * e, col = 16 * t + lr;
+          const bool ok = row < nr && col < R;
+          const T out = (T)acc[t][e];
+          if (ok) ((T*)a.factor)[(r0 + row) * R + col] = out;
+          const double o = ok ? (double)out : 0.0;
+          An[row * RP + col] = o;
+          ip = fma(Mr[row * RP + col], o, ip);
+        }
+      }
+    }
   }
   __syncthreads();
-  for (int e = tid; e < R * R; e += blockDim.x) {
-    const int r = e / R, s = e % R;
-    double acc0 = 0.0, acc1 = 0.0;
-    int i = 0;
-    for (; i + 1 < nr; i += 2) {
-      acc0 = fma(An[i * R + r], An[i * R + s], acc0);
-      acc1 = fma(An[(i + 1) * R + r], An[(i + 1) * R + s], acc1);
+  // Partial Gram A^T A of these rows (zero rows past nr), one 16 x 16 tile
+  // per wave and step
+  for (int tile = w; tile < RT * RT; tile += 4) {
+    const int tr = tile / RT, tc = tile % RT;
+    doublex4 g = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kSolveRows; k0 += 4)
+      g = mfma_f64(An[(k0 + lk) * RP + 16 * tr + lr], An[(k0 + lk) * RP + 16 * tc + lr], g);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = 16 * tr + lk + 4 * e, s = 16 * tc + lr;
+      if (r < R && s < R) a.gram_partial[(int64_t)blockIdx.x * R * R + r * R + s] = g[e];
     }
-    if (i < nr) acc0 = fma(An[i * R + r], An[i * R + s], acc0);
-    a.gram_partial[(int64_t)blockIdx.x * R * R + e] = acc0 + acc1;
   }
   if (a.ip_partial) {
     ip = wave_sum(ip);
@@ -402,12 +224,6 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   if (ticket != (int)gridDim.x - 1) return;
   __threadfence();
   const int nblk = gridDim.x;
-  for (int e = tid; e < R * R; e += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b)
-      s += __hip_atomic_load(a.gram_partial + (int64_t)b * R * R + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a.gram_out[e] = s;
-  }
   // Sum the per-block partials in block order.  Every block's partials were
   // published before its ticket (release fence above, acquire fence here);
   // this block never read them before, so plain loads see them.  Loads are
@@ -465,6 +281,15 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
       const double nx2 = *a.normx2;
       a.errors[a.err_slot] = sqrt(fabs(nx2 + m2 - 2.0 * ip_all)) / sqrt(nx2);
     }
+  }
+  if (a.next_mode >= 0) {
+    // V^-1 for the next solve, from the Grams with this update's (gram_out,
+    // written above by this block; the barrier orders it for every thread)
+    __syncthreads();
+    const int st = invert_v<256>(a, a.ndim, a.next_mode, a.mode, a.gram_out, a.weights, wdt, a.l2, R, Lbuf, Vi, pbuf,
+                                 pflag, &piv, &flag);
+    for (int e = tid; e < R * R; e += blockDim.x) a.vi_next[e] = Vi[(e / R) * RP + e % R];
+    if (tid == 0) *a.vist_next = st;
   }
   if (tid == 0) *a.counter = 0;
 }
@@ -555,7 +380,8 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
                          double l2_reg, const void* mttkrp, int64_t rows, int rank, void* factor_out,
                          double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
                          size_t ws_bytes, const double* normx2, double* errors, int err_slot,
-                         const double* vi_pre, const int* vi_status, void* stream) {
+                         const double* vi_pre, const int* vi_status, int next_mode, double* vi_next,
+                         int* vist_next, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_solve: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_solve: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_solve: mode %d", mode);
@@ -584,6 +410,11 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
   TLA_REQUIRE(!vi_pre || vi_status, "cp_solve: precomputed inverse without status");
   a.vi_pre = vi_pre;
   a.vi_status = vi_status;
+  TLA_REQUIRE(next_mode < ndim && (next_mode < 0 || (counter && vi_next && vist_next)),
+              "cp_solve: the next mode's inverse needs counter, vi_next and vist_next");
+  a.next_mode = next_mode < 0 ? -1 : next_mode;
+  a.vi_next = vi_next;
+  a.vist_next = vist_next;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TLA_F32) cp_solve_kernel<float><<<nblk, 256, 0, st>>>(a);
   else cp_solve_kernel<double><<<nblk, 256, 0, st>>>(a);
@@ -617,26 +448,6 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
                             size_t ws_bytes, void* stream) {
   return tla::cp_solve_launch(dtype, grams, ndim, mode, weights, l2_reg, mttkrp, rows, rank, factor_out, gram_out,
                               iprod_out, status_out, counter, ws, ws_bytes, nullptr, nullptr, 0, nullptr, nullptr,
-                              stream);
+                              -1, nullptr, nullptr, stream);
 }
 
-int tla::cp_inverse_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights,
-                           double l2_reg, int rank, double* vi_out, int* status_out, void* stream) {
-  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_inverse: bad dtype %d", dtype);
-  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder && mode >= 0 && mode < ndim, "cp_inverse: order %d mode %d", ndim,
-              mode);
-  TLA_REQUIRE(rank >= 1 && rank <= kMaxRank, "cp_inverse: rank %d", rank);
-  TLA_REQUIRE(vi_out && status_out, "cp_inverse: NULL output");
-  SolveArgs a{};
-  for (int m = 0; m < ndim; ++m) a.grams[m] = grams[m];
-  a.ndim = ndim;
-  a.mode = mode;
-  a.R = rank;
-  a.weights = weights;
-  a.l2 = l2_reg;
-  hipStream_t st = (hipStream_t)stream;
-  if (dtype == TLA_F32) cp_inverse_kernel<float><<<1, 256, 0, st>>>(a, vi_out, status_out);
-  else cp_inverse_kernel<double><<<1, 256, 0, st>>>(a, vi_out, status_out);
-  TLA_LAUNCH_CHECK();
-  return TLA_OK;
-}

@@ -43,6 +43,18 @@ constexpr int kMaxOrder = 8;
 constexpr int kMaxRank = 64;  // CP rank limit of the ALS solve / Gram kernels
 constexpr int kWave = 64;
 
+// A side job for a persistent MTTKRP launch: invert the normal-equations
+// matrix of `mode` (see inverse.h) in one extra workgroup.  R == 0: none.
+struct InvJob {
+  const double* grams;  // ndim Grams, R x R FP64, back to back
+  const void* weights;  // nullable; dtype wdt (TLA_F32 / TLA_F64)
+  int wdt;
+  double l2;
+  int ndim, mode, R;
+  double* vi;  // R x R FP64 out
+  int* vist;   // status out
+};
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -82,16 +94,25 @@ inline void strided_reduce(const float* in, int64_t groups, int64_t E, int64_t K
   strided_reduce_kernel<<<grid, 256, 0, st>>>(in, E, K, gs, ks, out);
 }
 
-// tla_cp_solve with, optionally, the relative error of the updated model
-// written to errors[err_slot] by the finalising block (last mode of a sweep).
+// tla_cp_solve with the driver's options: the relative error of the
+// updated model written to errors[err_slot] by the finalising block (last
+// mode of a sweep); V^-1 taken from vi_pre when *vi_status >= 0; and, with
+// next_mode >= 0, V^-1 of that mode (Grams including this update) written
+// to vi_next / vist_next (status 0 SPD elimination, 1 pivoting fallback, 2
+// singular) by the finalising block.
 int cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights, double l2_reg,
                     const void* mttkrp, int64_t rows, int rank, void* factor_out, double* gram_out, double* iprod_out,
                     int* status_out, int* counter, void* ws, size_t ws_bytes, const double* normx2, double* errors,
-                    int err_slot, const double* vi_pre, const int* vi_status, void* stream);
-// V^-1 of the solve for `mode` alone (one workgroup): vi_out R x R FP64,
-// status_out 0 / 1 (pivoting fallback) / 2 (singular).
-int cp_inverse_launch(int dtype, const double* const* grams, int ndim, int mode, const void* weights, double l2_reg,
-                      int rank, double* vi_out, int* status_out, void* stream);
+                    int err_slot, const double* vi_pre, const int* vi_status, int next_mode, double* vi_next,
+                    int* vist_next, void* stream);
+
+// tla_mttkrp (ke == kb + 1) / tla_mttkrp_partial keeping modes [kb, ke),
+// with an optional side job (InvJob, nullable) that the persistent kernels
+// run in one extra workgroup; *job_taken (nullable) reports whether this
+// launch ran it (otherwise the caller must do it some other way).
+int mttkrp_launch(int x_dtype, const void* X, int ndim, const int64_t* shape, const void* const* factors,
+                  const void* weights, int rank, int kb, int ke, int precision, void* out, void* ws, size_t ws_bytes,
+                  const InvJob* job, bool* job_taken, hipStream_t st);
 
 struct ModeShape {
   int ndim;

@@ -74,14 +74,12 @@ struct CpAls {
   int* counter;
   double* err_out;  // errors[] of the current sweep (or NULL): fused into the last solve
   int err_slot;
-  // V^-1 of a mode is computed on a second stream while that mode's tensor
-  // contraction runs (the Grams it needs are final by then)
+  // V^-1 of the next mode to solve, formed by the finalising block of the
+  // previous solve; vi_valid[m]: vi[m] was written since the Grams last
+  // changed outside a solve (host-side bookkeeping, stream-ordered)
   double* vi[kMaxOrder];
   int* vist;
-  hipStream_t aux;
-  hipEvent_t ev_fork;
-  hipEvent_t ev_inv[kMaxOrder];
-  bool inv_pending[kMaxOrder];
+  bool vi_valid[kMaxOrder];
   float* M;
   float* T1;
   float* T2;
@@ -147,50 +145,65 @@ static Layout layout(int x_dtype, int ndim, const int64_t* shape, int R, int pre
   return L;
 }
 
-// Start V_n^-1 on the second stream; the Grams it reads are final on `st`
-// at this point.
-static int fork_inverse(CpAls* h, int n, hipStream_t st) {
-  const double* gp[kMaxOrder];
-  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
-  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
-  TLA_HIP(hipEventRecord(h->ev_fork, st));
-  TLA_HIP(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
-  int rc = cp_inverse_launch(dt, gp, h->ndim, n, h->weights, h->l2, h->R, h->vi[n], h->vist + n, h->aux);
-  if (rc) return rc;
-  TLA_HIP(hipEventRecord(h->ev_inv[n], h->aux));
-  h->inv_pending[n] = true;
-  return TLA_OK;
+static void invalidate_inverses(CpAls* h) {
+  for (int m = 0; m < kMaxOrder; ++m) h->vi_valid[m] = false;
 }
 
-static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, hipStream_t st) {
-  const double* gp[kMaxOrder];
-  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
-  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
-  const bool pre = h->inv_pending[n];
-  if (pre) TLA_HIP(hipStreamWaitEvent(st, h->ev_inv[n], 0));
-  h->inv_pending[n] = false;
-  return cp_solve_launch(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n], h->grams[n],
-                         want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws, h->ws_bytes, h->normx2,
-                         want_iprod ? h->err_out : nullptr, h->err_slot, pre ? h->vi[n] : nullptr,
-                         pre ? h->vist + n : nullptr, st);
+// Where each solve's V^-1 comes from: the MTTKRP launch that precedes the
+// solve runs it in one extra workgroup (side job) when its kernel is a
+// persistent one; for the second mode of a dimension-tree half (only a short
+// rank-diagonal contraction in between) the previous solve's finalising
+// block forms it; otherwise the solve inverts V itself.
+static InvJob inv_job(const CpAls* h, int n) {
+  InvJob j{};
+  j.grams = h->grams[0];  // the Grams are back to back in the workspace
+  j.weights = h->weights;
+  j.wdt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  j.l2 = h->l2;
+  j.ndim = h->ndim;
+  j.mode = n;
+  j.R = h->R;
+  j.vi = h->vi[n];
+  j.vist = h->vist + n;
+  return j;
 }
 
-static int mttkrp_mode(CpAls* h, int n, hipStream_t st) {
+// MTTKRP keeping modes [kb, ke) into out, with the side job for mode
+// job_mode (-1: none).
+static int mttkrp_with_job(CpAls* h, int kb, int ke, void* out, int job_mode, hipStream_t st) {
   const void* fp[kMaxOrder];
   for (int m = 0; m < h->ndim; ++m) fp[m] = h->factors[m];
-  return tla_mttkrp(h->x_dtype, h->X, h->ndim, h->shape, fp, h->weights, h->R, n, h->precision, h->M,
-                    h->ws, h->ws_bytes, st);
+  const InvJob job = job_mode >= 0 ? inv_job(h, job_mode) : InvJob{};
+  bool taken = false;
+  const int rc = mttkrp_launch(h->x_dtype, h->X, h->ndim, h->shape, fp, h->weights, h->R, kb, ke, h->precision, out,
+                               h->ws, h->ws_bytes, job_mode >= 0 ? &job : nullptr, &taken, st);
+  if (rc == TLA_OK && taken) h->vi_valid[job_mode] = true;
+  return rc;
+}
+
+// Solve mode n; next >= 0: the finalising block also forms V^-1 of mode next.
+static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, int next, hipStream_t st) {
+  const double* gp[kMaxOrder];
+  for (int m = 0; m < h->ndim; ++m) gp[m] = h->grams[m];
+  const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  const bool pre = h->vi_valid[n];
+  invalidate_inverses(h);  // this update changes G_n
+  const int rc = cp_solve_launch(dt, gp, h->ndim, n, h->weights, h->l2, M, h->shape[n], h->R, h->factors[n],
+                                 h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws,
+                                 h->ws_bytes, h->normx2, want_iprod ? h->err_out : nullptr, h->err_slot,
+                                 pre ? h->vi[n] : nullptr, pre ? h->vist + n : nullptr, next,
+                                 next >= 0 ? h->vi[next] : nullptr, next >= 0 ? h->vist + next : nullptr, st);
+  if (rc == TLA_OK && next >= 0) h->vi_valid[next] = true;
+  return rc;
 }
 
 static int sweep_standard(CpAls* h, hipStream_t st) {
   const int N = h->ndim;
   for (int n = 0; n < N; ++n) {
     if (h->fixed[n]) continue;
-    int rc = fork_inverse(h, n, st);
+    int rc = mttkrp_with_job(h, n, n + 1, h->M, n, st);
     if (rc) return rc;
-    rc = mttkrp_mode(h, n, st);
-    if (rc) return rc;
-    rc = solve_mode(h, n, h->M, n == N - 1, st);
+    rc = solve_mode(h, n, h->M, n == N - 1, -1, st);
     if (rc) return rc;
   }
   return TLA_OK;
@@ -198,65 +211,45 @@ static int sweep_standard(CpAls* h, hipStream_t st) {
 
 static int sweep_dimtree(CpAls* h, hipStream_t st) {
   const int N = h->ndim;
-  const void* fp[kMaxOrder];
   const int R = h->R;
   // First half: modes 0 and 1 share T1 = X x_{m>=2} A_m.
   if (!(h->fixed[0] && h->fixed[1])) {
-    for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
-    if (!h->fixed[0]) {
-      int rc = fork_inverse(h, 0, st);
-      if (rc) return rc;
-    }
-    int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 0, 2, h->precision, h->T1,
-                                h->ws, h->ws_bytes, st);
+    int rc = mttkrp_with_job(h, 0, 2, h->T1, h->fixed[0] ? -1 : 0, st);
     if (rc) return rc;
     if (!h->fixed[0]) {
       rc = tla_rankdiag_contract(h->T1, h->shape[0], h->shape[1], R, (const float*)h->factors[1], 1, h->M,
                                  h->ws, h->ws_bytes, st);
       if (rc) return rc;
-      rc = solve_mode(h, 0, h->M, false, st);
+      rc = solve_mode(h, 0, h->M, false, h->fixed[1] ? -1 : 1, st);
       if (rc) return rc;
     }
     if (!h->fixed[1]) {
-      // V_1 needs the Gram of the mode-0 factor just updated; only the short
-      // rank-diagonal contraction could hide it, so it is computed inside the
-      // solve instead of on the second stream
       rc = tla_rankdiag_contract(h->T1, h->shape[0], h->shape[1], R, (const float*)h->factors[0], 0, h->M,
                                  h->ws, h->ws_bytes, st);
       if (rc) return rc;
-      rc = solve_mode(h, 1, h->M, false, st);
+      rc = solve_mode(h, 1, h->M, false, -1, st);
       if (rc) return rc;
     }
   }
   if (N == 3) {
-    int rc = fork_inverse(h, 2, st);
+    int rc = mttkrp_with_job(h, 2, 3, h->M, 2, st);
     if (rc) return rc;
-    rc = mttkrp_mode(h, 2, st);
-    if (rc) return rc;
-    return solve_mode(h, 2, h->M, true, st);
+    return solve_mode(h, 2, h->M, true, -1, st);
   }
   // N == 4: modes 2 and 3 share T2 = X x_0 A_0 x_1 A_1 (updated factors).
-  for (int m = 0; m < N; ++m) fp[m] = h->factors[m];
-  if (!h->fixed[2]) {
-    int rc = fork_inverse(h, 2, st);
-    if (rc) return rc;
-  }
-  int rc = tla_mttkrp_partial(h->x_dtype, h->X, N, h->shape, fp, h->weights, R, 2, 4, h->precision, h->T2,
-                              h->ws, h->ws_bytes, st);
+  int rc = mttkrp_with_job(h, 2, 4, h->T2, h->fixed[2] ? 3 : 2, st);
   if (rc) return rc;
   if (!h->fixed[2]) {
     rc = tla_rankdiag_contract(h->T2, h->shape[2], h->shape[3], R, (const float*)h->factors[3], 1, h->M, h->ws,
                                h->ws_bytes, st);
     if (rc) return rc;
-    rc = solve_mode(h, 2, h->M, false, st);
+    rc = solve_mode(h, 2, h->M, false, 3, st);
     if (rc) return rc;
   }
-  rc = fork_inverse(h, 3, st);
-  if (rc) return rc;
   rc = tla_rankdiag_contract(h->T2, h->shape[2], h->shape[3], R, (const float*)h->factors[2], 0, h->M, h->ws,
                              h->ws_bytes, st);
   if (rc) return rc;
-  return solve_mode(h, 3, h->M, true, st);
+  return solve_mode(h, 3, h->M, true, -1, st);
 }
 
 }  // namespace tla
@@ -315,14 +308,7 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   h->counter = h->status + 1;  // the status slot is 8 bytes: int status, int block counter
   for (int m = 0; m < ndim; ++m) h->vi[m] = (double*)(base + L.vi) + size_t(m) * rank * rank;
   h->vist = (int*)(base + L.vist);
-  bool ok = hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
-  for (int m = 0; ok && m < ndim; ++m) ok = hipEventCreateWithFlags(&h->ev_inv[m], hipEventDisableTiming) == hipSuccess;
-  if (!ok) {
-    tla_cp_als_destroy(h);
-    set_error("cp_als: could not create the auxiliary stream / events");
-    return TLA_ERR_HIP;
-  }
+  invalidate_inverses(h);
   h->M = (float*)(base + L.M);
   h->T1 = dimtree ? (float*)(base + L.T1) : nullptr;
   h->T2 = (dimtree && ndim == 4) ? (float*)(base + L.T2) : nullptr;
@@ -338,13 +324,6 @@ extern "C" int tla_cp_als_dimtree_active(void* handle) { return handle ? ((CpAls
 extern "C" void tla_cp_als_destroy(void* handle) {
   CpAls* h = (CpAls*)handle;
   if (!h) return;
-  if (h->aux) {
-    (void)hipStreamSynchronize(h->aux);
-    (void)hipStreamDestroy(h->aux);
-  }
-  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-  for (int m = 0; m < h->ndim; ++m)
-    if (h->ev_inv[m]) (void)hipEventDestroy(h->ev_inv[m]);
   delete h;
 }
 
@@ -355,6 +334,7 @@ extern "C" int tla_cp_als_refresh(void* handle, int with_norm, void* stream) {
   TLA_REQUIRE(h, "cp_als: NULL handle");
   hipStream_t st = (hipStream_t)stream;
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
+  invalidate_inverses(h);
   zero_ints_kernel<<<1, 64, 0, st>>>(h->status, 2);
   TLA_LAUNCH_CHECK();
   if (with_norm) {
@@ -401,6 +381,7 @@ extern "C" int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int
       GramPtrs g{};
       for (int m = 0; m < h->ndim; ++m) g.p[m] = h->grams[m];
       TLA_REQUIRE(h->weights, "cp_als: normalize_factors needs a weights vector");
+      invalidate_inverses(h);
       normalize_scales_kernel<<<1, 64, 0, st>>>(g, h->ndim, h->R, h->weights, h->scales);
       TLA_LAUNCH_CHECK();
       for (int m = 0; m < h->ndim; ++m) {
@@ -571,7 +552,8 @@ struct GramPtrArray {
 
 // A_prev of the next pair is the factor kept here, written in the same pass.
 __global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, FactorPtrs Aprev, GramPtrArray grams,
-                                        const double* gsave, int ndim, int R, double* errors, int* accepted, int s) {
+                                        const double* gsave, int ndim, int R, double* errors, int* accepted, int s,
+                                        int* vist) {
   const bool ok = errors[s + 1] < errors[s];
   const int m = blockIdx.y;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -587,6 +569,7 @@ __global__ void accept_or_revert_kernel(FactorPtrs A, FactorPtrs Asave, FactorPt
       accepted[s + 1] = ok ? 1 : 0;
       if (!ok) errors[s + 1] = errors[s];
     }
+    if (!ok && threadIdx.x < ndim) vist[threadIdx.x] = -1;
   }
 }
 
@@ -690,8 +673,12 @@ extern "C" int tla_cp_als_run_extrapolated(void* handle, int n_sweeps, const flo
     ea.slot = s + 1;
     extrapolate_gram_kernel<<<egrid, 256, 0, st>>>(ea);
     TLA_LAUNCH_CHECK();
+    invalidate_inverses(h);
     if ((rc = sweep(s + 1))) return rc;
-    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, Aprev, gpa, ea.gsave, h->ndim, R, errors, accepted, s);
+    // on a revert the kernel also marks the inverse formed for the next
+    // sweep (from the rejected Grams) as stale
+    accept_or_revert_kernel<<<fgrid, 256, 0, st>>>(A, Asave, Aprev, gpa, ea.gsave, h->ndim, R, errors, accepted, s,
+                                                   h->vist);
     TLA_LAUNCH_CHECK();
     s += 2;
   }

@@ -34,6 +34,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "inverse.h"
 
 namespace tla {
 
@@ -507,6 +508,7 @@ struct StreamArgs {
   const float* Fk;
   const float* Sq;  // (ntiles / (J / 16)) x R block scales, precomputed
   float* out;
+  InvJob inv;  // optional side job for the grid's last workgroup
 };
 
 template <int RT>
@@ -634,6 +636,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   constexpr int RP = RT * 16;
   constexpr int NT = 64 * kStreamWaves;
   extern __shared__ __align__(16) unsigned char smem[];
+  if (take_inv_job<NT>(args.inv, (char*)smem)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, g = lane >> 4;
   const int R = args.R;
@@ -661,7 +664,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   }
   __syncthreads();
 
-  const int64_t W = (int64_t)gridDim.x * kStreamWaves;
+  const int64_t W = (int64_t)(gridDim.x - (args.inv.R ? 1 : 0)) * kStreamWaves;
   const int64_t w = (int64_t)blockIdx.x * kStreamWaves + wave;
   const XT* X = (const XT*)args.X;
   const int64_t tpq = args.J / 16;
@@ -928,6 +931,7 @@ struct LastStreamArgs {
   int64_t G;        // blocks per column group
   int64_t nvisits;  // ceil(P / (32 * rsplit))
   int ldT;          // > 0: F_j staged in LDS transposed, [RT * 16][ldT]
+  InvJob inv;       // optional side job for the grid's last workgroup
 };
 
 // One visit's operands: 8 rows of this lane's column quad plus the Khatri-Rao
@@ -1067,7 +1071,8 @@ __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs 
   const int64_t K = args.K;
   const int64_t cgroup = blockIdx.x / args.G;
   const int64_t b = blockIdx.x - cgroup * args.G;
-  extern __shared__ float fjs[];  // FJL: F_j transposed, [RT * 16][ldT]
+  extern __shared__ __align__(16) float fjs[];  // FJL: F_j transposed, [RT * 16][ldT]
+  if (take_inv_job<512>(args.inv, (char*)fjs)) return;
   if constexpr (FJL) {
     const int64_t J = args.J;
     const int ldT = args.ldT;
@@ -1212,8 +1217,10 @@ static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Plan the contraction that keeps the contiguous block of modes
 // [kb, ke) and contracts every other mode with its factor.  Standard MTTKRP
 // of mode n is kb = n, ke = n + 1.
+// reserve_block: leave one workgroup slot of the 256 free for a side job
+// (InvJob) in the persistent kernels.
 static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb, int ke,
-                     int precision, Plan* p) {
+                     int precision, Plan* p, bool reserve_block = false) {
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "tensor order %d not in [2, %d]", ndim, kMaxOrder);
   TLA_REQUIRE(R >= 1, "rank must be >= 1");
   TLA_REQUIRE(0 <= kb && kb < ke && ke <= ndim, "bad kept-mode range [%d, %d)", kb, ke);
@@ -1261,7 +1268,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
         s.ntiles = Prow / 16;
         const int64_t tpq = J / 16;
         constexpr int NW = kStreamWaves;
-        int64_t blocks = 256;
+        int64_t blocks = reserve_block ? 255 : 256;
         if (const char* e = getenv("TLA_STREAM_BLOCKS")) blocks = std::max(1, atoi(e));
         blocks = std::min<int64_t>(blocks, (s.ntiles + NW - 1) / NW);
         int64_t Wv = blocks * NW;
@@ -1406,7 +1413,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     s.rsplit = 8 / wpr;
     const int64_t cgroups = (wpr == 8) ? ceil_div(ktiles, 8) : 1;
     s.nvisits = (P + 32 * s.rsplit - 1) / (32 * s.rsplit);
-    int64_t target = 256;
+    int64_t target = reserve_block ? 255 : 256;
     if (const char* e = getenv("TLA_LAST_STREAM_BLOCKS")) target = std::max(1, atoi(e));
     int64_t G = std::max<int64_t>(1, target / cgroups);
     G = std::min<int64_t>(G, s.nvisits);
@@ -1495,9 +1502,10 @@ static int allow_lds(F* fn, size_t bytes) {
 template <int PREC, typename XT, int RT, int EPI>
 static int launch_stream_rt(const Plan& p, hipStream_t st) {
   auto* fn = &mttkrp_stream_kernel<PREC, XT, RT, EPI>;
-  int rc = allow_lds(fn, p.smem);
+  const size_t lds = p.st.inv.R ? std::max(p.smem, inv_job_lds_bytes(p.st.inv.R)) : p.smem;
+  int rc = allow_lds(fn, lds);
   if (rc) return rc;
-  fn<<<p.grid, 64 * kStreamWaves, p.smem, st>>>(p.st);
+  fn<<<p.grid, 64 * kStreamWaves, lds, st>>>(p.st);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }
@@ -1525,13 +1533,17 @@ static int launch_stream(const Plan& p, hipStream_t st) {
 template <int PREC, typename XT, int RT>
 static int launch_last_stream_rt(const Plan& p, hipStream_t st) {
   const LastStreamArgs a = p.ls;
+  const size_t lds = a.inv.R ? std::max(p.smem, inv_job_lds_bytes(a.inv.R)) : p.smem;
   if (a.ldT > 0) {
     auto* fn = &mttkrp_last_stream_kernel<PREC, XT, RT, true>;
-    const int rc = allow_lds(fn, p.smem);
+    const int rc = allow_lds(fn, lds);
     if (rc) return rc;
-    fn<<<p.grid, 512, p.smem, st>>>(a);
+    fn<<<p.grid, 512, lds, st>>>(a);
   } else {
-    mttkrp_last_stream_kernel<PREC, XT, RT, false><<<p.grid, 512, 0, st>>>(a);
+    auto* fn = &mttkrp_last_stream_kernel<PREC, XT, RT, false>;
+    const int rc = allow_lds(fn, lds);
+    if (rc) return rc;
+    fn<<<p.grid, 512, lds, st>>>(a);
   }
   TLA_LAUNCH_CHECK();
   return TLA_OK;
@@ -1600,8 +1612,18 @@ static int launch_generic(const T* X, int ndim, const int64_t* shape, const void
 
 static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const int64_t* shape,
                     const void* const* factors, const void* weights, int R, int kb, int ke,
-                    int precision, void* out, void* ws, size_t ws_bytes, hipStream_t st) {
+                    int precision, void* out, void* ws, size_t ws_bytes, hipStream_t st,
+                    const InvJob* job = nullptr, bool* job_taken = nullptr) {
   Plan p = p0;
+  if (job_taken) *job_taken = false;
+  // the side job rides along as one extra workgroup when the grid leaves a
+  // slot of the 256 free (all workgroups resident at once)
+  if (job && job->R > 0 && (p.path == Path::Stream || p.path == Path::LastStream) && p.grid.x + 1 <= 256) {
+    if (p.path == Path::Stream) p.st.inv = *job;
+    else p.ls.inv = *job;
+    p.grid.x += 1;
+    if (job_taken) *job_taken = true;
+  }
   if (p.path == Path::Generic) {
     if (x_dtype == TLA_F64)
       return launch_generic<double, double>((const double*)X, ndim, shape, factors, weights, R, kb,
@@ -1749,6 +1771,21 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
 }  // namespace tla
 
 using namespace tla;
+
+int tla::mttkrp_launch(int x_dtype, const void* X, int ndim, const int64_t* shape, const void* const* factors,
+                       const void* weights, int rank, int kb, int ke, int precision, void* out, void* ws,
+                       size_t ws_bytes, const InvJob* job, bool* job_taken, hipStream_t st) {
+  if (job_taken) *job_taken = false;
+  TLA_REQUIRE(X && out && factors, "NULL pointer argument");
+  if (x_dtype == TLA_F64 && precision != TLA_PREC_VALU) precision = TLA_PREC_VALU;
+  Plan p;
+  int rc = make_plan(x_dtype, ndim, shape, rank, kb, ke, precision, &p, job != nullptr && job->R > 0);
+  if (rc != TLA_OK) return rc;
+  TLA_REQUIRE(ke == kb + 1 || p.path != Path::Generic, "partial MTTKRP needs an MFMA-eligible shape");
+  if (job) TLA_REQUIRE(job->R >= 0 && job->R <= kMaxRank && job->grams && job->vi && job->vist, "bad side job");
+  return run_plan(p, x_dtype, X, ndim, shape, factors, weights, rank, kb, ke, precision, out, ws, ws_bytes, st, job,
+                  job_taken);
+}
 
 extern "C" size_t tla_mttkrp_workspace_size(int x_dtype, int ndim, const int64_t* shape, int rank,
                                             int mode, int precision) {

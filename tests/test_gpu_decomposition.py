@@ -49,6 +49,26 @@ def test_parafac_float32_vs_oracle(cuda, prec, dimtree):
     assert np.max(np.abs(np.array(errs) - np.array(ref_errs))) < 1e-5, (prec, dimtree)
 
 
+@pytest.mark.parametrize("shape", [(64, 48, 128), (16, 32, 16, 64)])
+@pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("bf16x3", True), ("fp32", True)])
+@pytest.mark.parametrize("rank", [8, 40])
+def test_parafac_persistent_kernels_vs_oracle(cuda, shape, prec, dimtree, rank):
+    """Shapes whose MTTKRPs all run on the persistent streaming kernels, so
+    every solve takes V^-1 from the side workgroup of the preceding MTTKRP
+    launch or from the previous solve's finalising block: per-sweep err^2
+    within 1e-6 of the FP64 oracle on the same data and init."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(rank + len(shape))
+    f_true = [rng.random((s, rank)) for s in shape]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(rank), f_true, dtype=np.float32), 0.01, rng)
+    (_, _), ref_errs = T.parafac(x.astype(np.float64), rank, n_iter_max=12, init="random", random_state=5, tol=0,
+                                 return_errors=True)
+    _, errs = tla.parafac(g(x, cuda, torch.float32), rank, n_iter_max=12, init="random", random_state=5, tol=0,
+                          return_errors=True, precision=prec, dimtree=dimtree)
+    d = np.abs(np.array(errs) ** 2 - np.array(ref_errs) ** 2)
+    assert np.max(d) < 1e-6, (prec, dimtree, d)
+
+
 def test_parafac_order4_dimtree(cuda):
     import tensorly_amd as tla
     rng = np.random.default_rng(5)
