@@ -6,17 +6,18 @@
 // A are the eigenvectors of A A^T and the singular values the square roots
 // of its eigenvalues.
 //
-//  * n <= 64: cyclic Jacobi on the whole matrix in LDS (round-robin pair
-//    ordering, all n/2 rotations of a round applied in parallel).
-//  * n  > 64: block subspace iteration with Rayleigh-Ritz.  Block b = 64
-//    for k <= 56 (Ritz matrix and rotations in LDS); otherwise b =
-//    min(n, 128) for k <= 120 or n <= 128 (Ritz matrix in LDS, rotations in
-//    the workspace; b = n makes the first Rayleigh-Ritz step exact).  Each
-//    step: Z = G V (FP64), H = V^T Z, Jacobi(H) = W diag(t) W^T (warm: the
-//    previous Ritz basis nearly diagonalises H), residuals of the k leading
-//    Ritz pairs, V <- orth(Z W) by shifted Cholesky-QR (three passes).
+//  * n <= 128: cyclic Jacobi on the whole matrix (round-robin pair
+//    ordering, all n/2 rotations of a round applied in parallel); the matrix
+//    in LDS with an odd leading dimension, the rotations in LDS (n <= 64) or
+//    in the workspace.
+//  * n  > 128: block subspace iteration with Rayleigh-Ritz, block b = 64 for
+//    k <= 56 or 128 for k <= 120.  Each step: Z = G V, H = V^T Z on FP64
+//    MFMA, Jacobi(H) = W diag(t) W^T (warm: the previous Ritz basis nearly
+//    diagonalises H), Ritz vectors Y = V W and U = Z W = G Y, residuals of
+//    the k leading pairs, V <- orth(U) by two shifted Cholesky-QR passes
+//    (S = M^T M on MFMA, forward elimination of [S | I] in registers).
 //    Converged when every leading residual is below tol * t_1.
-// One workgroup per problem; the n x b blocks live in the caller's
+// One workgroup per problem; the padded np x b blocks live in the caller's
 // workspace (L2-resident).
 #include <math.h>
 
@@ -27,16 +28,54 @@
 namespace tla {
 
 constexpr int kEigThreads = 512;
-constexpr int kJac = 64;          // max Jacobi size (LDS)
+constexpr int kEigWaves = kEigThreads / 64;
+constexpr int kJac = 64;           // n <= kJac: Jacobi with A and W in LDS
+constexpr int kJacLarge = 128;     // n <= kJacLarge: Jacobi with A in LDS, W in the workspace
 constexpr int kEigCapLarge = 128;  // larger subspace block
 
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// D = A B + C on FP64 MFMA (16 x 4 by 4 x 16): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds D[l / 16 + 4 e][l % 16], e < 4.
+__device__ __forceinline__ doublex4 mfma_f64(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// C (M x N, ldc) = op(A) B over K, op(A)[m][k] = TA ? A[k * lda + m] : A[m * lda + k],
+// B[k][n] = B[k * ldb + n]; M, N multiples of 16, K a multiple of 8.  The
+// block's waves take 16 x 16 output tiles in turn; operands may be in LDS or
+// global memory (L2-resident here).  Ends with a barrier.
+template <bool TA>
+__device__ void block_gemm_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N,
+                               int K) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const int tn = N / 16, ntile = (M / 16) * tn;
+  for (int t = wave; t < ntile; t += blockDim.x >> 6) {
+    const int mi = (t / tn) * 16, nj = (t % tn) * 16;
+    doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      // both k-steps' operands requested before the two MFMAs
+      const int ka = k0 + lk, kb = k0 + 4 + lk;
+      const double a0 = TA ? A[ka * lda + mi + lr] : A[(mi + lr) * lda + ka];
+      const double a1 = TA ? A[kb * lda + mi + lr] : A[(mi + lr) * lda + kb];
+      const double b0 = B[ka * ldb + nj + lr];
+      const double b1 = B[kb * ldb + nj + lr];
+      acc = mfma_f64(a0, b0, acc);
+      acc = mfma_f64(a1, b1, acc);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) C[(mi + lk + 4 * e) * ldc + nj + lr] = acc[e];
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-// Jacobi on an m x m symmetric matrix A (row-major, leading dimension LD),
-// accumulating rotations into W (same layout, initialised by the caller).
-// A lives in LDS; W may be in LDS or in global memory (the LD = 128 case,
-// where A alone takes 128 KB of LDS).  Returns the number of sweeps.
+// Jacobi on an m x m symmetric matrix A (LDS, leading dimension LDA: odd, so
+// that the column walks of a rotation hit distinct banks), accumulating the
+// rotations into W (leading dimension LDW, in LDS or global memory,
+// initialised by the caller).  Returns the number of sweeps.
 // ---------------------------------------------------------------------------
-template <int LD>
+template <int LDA, int LDW>
 __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int me = m + (m & 1);  // even number of slots (one dummy)
@@ -46,7 +85,7 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < m * m; e += nt) {
       const int i = e / m, j = e % m;
-      const double v = A[i * LD + j];
+      const double v = A[i * LDA + j];
       tot += v * v;
       if (i != j) off += v * v;
     }
@@ -72,9 +111,9 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         if (p > q) { const int t = p; p = q; q = t; }
         double c = 1.0, s = 0.0;
         if (q < m) {
-          const double apq = A[p * LD + q];
+          const double apq = A[p * LDA + q];
           if (fabs(apq) > 1e-300) {
-            const double app = A[p * LD + p], aqq = A[q * LD + q];
+            const double app = A[p * LDA + p], aqq = A[q * LDA + q];
             const double tau = (aqq - app) / (2.0 * apq);
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -94,25 +133,25 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         const int p = pr[2 * a], q = pr[2 * a + 1];
         if (q < m) {
           const double c = cs[2 * a], s = cs[2 * a + 1];
-          const double x = A[p * LD + j], y = A[q * LD + j];
-          A[p * LD + j] = c * x - s * y;
-          A[q * LD + j] = s * x + c * y;
+          const double x = A[p * LDA + j], y = A[q * LDA + j];
+          A[p * LDA + j] = c * x - s * y;
+          A[q * LDA + j] = s * x + c * y;
         }
       }
       __syncthreads();
       // columns of A and W
       for (int e = tid; e < npairs * m; e += nt) {
-        const int a = e / m, i = e % m;
+        const int a = e % npairs, i = e / npairs;
         const int p = pr[2 * a], q = pr[2 * a + 1];
         if (q < m) {
           const double c = cs[2 * a], s = cs[2 * a + 1];
-          double x = A[i * LD + p], y = A[i * LD + q];
-          A[i * LD + p] = c * x - s * y;
-          A[i * LD + q] = s * x + c * y;
-          x = W[i * LD + p];
-          y = W[i * LD + q];
-          W[i * LD + p] = c * x - s * y;
-          W[i * LD + q] = s * x + c * y;
+          double x = A[i * LDA + p], y = A[i * LDA + q];
+          A[i * LDA + p] = c * x - s * y;
+          A[i * LDA + q] = s * x + c * y;
+          x = W[i * LDW + p];
+          y = W[i * LDW + q];
+          W[i * LDW + p] = c * x - s * y;
+          W[i * LDW + q] = s * x + c * y;
         }
       }
       __syncthreads();
@@ -122,14 +161,14 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
 }
 
 // order[r] = index of the r-th largest diagonal entry of A (ties by index)
-template <int LD>
+template <int LDA>
 __device__ void sort_desc(const double* A, int m, int* order) {
   const int tid = threadIdx.x;
   if (tid < m) {
-    const double v = A[tid * LD + tid];
+    const double v = A[tid * LDA + tid];
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double u = A[j * LD + j];
+      const double u = A[j * LDA + j];
       if (u > v || (u == v && j < tid)) ++rank;
     }
     order[rank] = tid;
@@ -148,12 +187,14 @@ struct EigArgs {
   double tol;
   int max_iter;
   int* iters;
-  int b;      // subspace block: columns in use, <= the kernel's capacity
-  double* V;  // n x capacity
-  double* Z;  // n x capacity
-  double* Y;  // n x capacity
-  double* U;  // n x capacity
-  double* W;  // capacity x capacity (Ritz rotations; used when capacity = 128)
+  int np;     // n rounded up to a multiple of 16 (subspace kernel)
+  double* Gp;  // np x np zero-padded copy of G (subspace kernel)
+  double* V;   // np x capacity, likewise Z, Y, U
+  double* Z;
+  double* Y;
+  double* U;
+  double* W;  // rotations when they do not fit LDS
+  double* T;  // orthonormalising factor when it does not fit LDS
 };
 
 __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
@@ -162,163 +203,167 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
   return (double)h * (1.0 / 4294967296.0) - 0.5;
 }
 
-// Shifted Cholesky-QR of the n x b block M (global, row-major ld LD) in
-// place; S (LDS, ld LD) scratch.
-template <int LD>
-__device__ void block_orth(double* M, int n, int b, double* S, double* scratch, int* flag) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int pass = 0; pass < 3; ++pass) {
-    // S = M^T M
-    for (int e = tid; e < b * b; e += nt) {
-      const int i = e / b, j = e % b;
-      if (j < i) continue;
-      double s = 0.0;
-      for (int r = 0; r < n; ++r) s = fma(M[r * LD + i], M[r * LD + j], s);
-      S[i * LD + j] = s;
-      S[j * LD + i] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double tr = 0.0;
-      for (int i = 0; i < b; ++i) tr = fmax(tr, S[i * LD + i]);
-      scratch[40] = tr;
-    }
-    __syncthreads();
-    const double shift = (pass == 0) ? 1e-13 * scratch[40] * (double)n : 0.0;
-    // Cholesky S + shift I = R^T R (R upper, stored in S upper)
-    if (tid == 0) *flag = 0;
-    __syncthreads();
-    for (int j = 0; j < b; ++j) {
-      if (tid == 0) {
-        double d = S[j * LD + j] + shift;
-        for (int l = 0; l < j; ++l) d -= S[l * LD + j] * S[l * LD + j];
-        if (!(d > 0.0)) { d = 1e-300; *flag = 1; }
-        S[j * LD + j] = sqrt(d);
-      }
-      __syncthreads();
-      for (int i = j + 1 + tid; i < b; i += nt) {
-        double v = S[j * LD + i];
-        for (int l = 0; l < j; ++l) v -= S[l * LD + j] * S[l * LD + i];
-        S[j * LD + i] = v / S[j * LD + j];
-      }
-      __syncthreads();
-    }
-    // M = M R^-1 (row-wise forward substitution)
-    for (int r = tid; r < n; r += nt) {
-      double* row = M + r * LD;
-      for (int j = 0; j < b; ++j) {
-        double v = row[j];
-        for (int l = 0; l < j; ++l) v -= row[l] * S[l * LD + j];
-        row[j] = v / S[j * LD + j];
-      }
-    }
-    __syncthreads();
-  }
-}
-
+// Jacobi on the whole matrix: CAP 64 keeps A and W in LDS; CAP 128 keeps A
+// in LDS and W in the workspace (a.W, ld 128).
+template <int CAP>
 __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigArgs a) {
-  __shared__ double A[kJac * kJac];
-  __shared__ double W[kJac * kJac];
-  __shared__ double cs[kJac + 2];
-  __shared__ int pr[kJac + 2];
+  constexpr int LDA = CAP + 1;
+  __shared__ double A[CAP * LDA];
+  __shared__ double Wl[CAP == kJac ? CAP * LDA : 1];
+  __shared__ double cs[CAP + 2];
+  __shared__ int pr[CAP + 2];
   __shared__ double scratch[64];
-  __shared__ int order[kJac];
+  __shared__ int order[CAP];
+  constexpr int LDW = CAP == kJac ? LDA : CAP;
+  double* W = CAP == kJac ? Wl : a.W;
   const int n = a.n, tid = threadIdx.x;
   for (int e = tid; e < n * n; e += blockDim.x) {
     const int i = e / n, j = e % n;
-    A[i * kJac + j] = 0.5 * (a.G[i * n + j] + a.G[j * n + i]);
-    W[i * kJac + j] = (i == j) ? 1.0 : 0.0;
+    A[i * LDA + j] = 0.5 * (a.G[i * n + j] + a.G[j * n + i]);
+    W[i * LDW + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int sweeps = lds_jacobi<kJac>(A, W, n, 60, cs, pr, scratch);
-  sort_desc<kJac>(A, n, order);
+  const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, pr, scratch);
+  sort_desc<LDA>(A, n, order);
   for (int e = tid; e < n * a.k; e += blockDim.x) {
     const int i = e / a.k, r = e % a.k;
-    a.evecs[i * a.k + r] = W[i * kJac + order[r]];
+    a.evecs[i * a.k + r] = W[i * LDW + order[r]];
   }
-  if (tid < a.k) a.evals[tid] = A[order[tid] * kJac + order[tid]];
+  for (int r = tid; r < a.k; r += blockDim.x) a.evals[r] = A[order[r] * LDA + order[r]];
   if (tid == 0 && a.iters) *a.iters = sweeps;
 }
 
-// Block subspace iteration with Rayleigh-Ritz.  CAP is the block capacity:
-// 64 keeps H and the rotations W in LDS; 128 keeps H (128 KB) in LDS and W
-// in the workspace.  a.b <= CAP columns are used; with a.b == n the first
-// Rayleigh-Ritz step is already exact.
-template <int CAP>
+// Out = M T with T = L^-T D^-1/2 from S + shift I = L D L^T, S = M^T M: the
+// Cholesky-QR orthonormalisation of the np x b block M (global, ld CAP),
+// through a forward elimination of [S | I] held in registers by the block
+// (TPR threads per row; per pivot its row and column go through LDS, one
+// barrier).  S and the factor live in Sbuf (LDS, ld b + 1) and Tbuf (ld ldt).
+template <int B>
+__device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, double* Tbuf, int ldt, bool shift,
+                           double* pbuf, double* dpiv, double* scratch) {
+  constexpr int LDS_S = B + 1, CAP = B;
+  constexpr int TPR = kEigThreads / B, CPT = 2 * B / TPR;
+  block_gemm_f64<true>(M, CAP, M, CAP, Sbuf, LDS_S, B, B, np);
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < B; ++i) mx = fmax(mx, Sbuf[i * LDS_S + i]);
+    scratch[40] = mx;
+  }
+  __syncthreads();
+  const double mx = scratch[40];
+  const double sh = shift ? 1e-13 * mx * (double)np : 0.0;
+  const double floor = 1e-300 + 1e-30 * mx;
+  const int i = tid / TPR, seg = tid % TPR, c0 = seg * CPT;
+  double x[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = c0 + j;
+    x[j] = c < B ? Sbuf[i * LDS_S + c] + (c == i ? sh : 0.0) : ((c - B == i) ? 1.0 : 0.0);
+  }
+  for (int kb = 0; kb < TPR / 2; ++kb) {
+#pragma unroll
+    for (int kk = 0; kk < CPT; ++kk) {
+      const int k = kb * CPT + kk;
+      double* row = pbuf + (k & 1) * (3 * B);
+      double* col = row + 2 * B;
+      if (i == k) {
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) row[c0 + j] = x[j];
+      }
+      if (seg == kb) col[i] = x[kk];
+      __syncthreads();
+      const double p = fmax(col[k], floor);
+      if (i == k && seg == 0) dpiv[k] = p;
+      if (i > k) {
+        const double f = col[i] / p;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) x[j] = fma(-f, row[c0 + j], x[j]);
+      }
+    }
+  }
+  __syncthreads();
+  // x now holds row i of [D L^T | L^-1]; T[r][i] = L^-1[i][r] / sqrt(D[i])
+  const double ds = 1.0 / sqrt(dpiv[i]);
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = c0 + j;
+    if (c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds;
+  }
+  __syncthreads();
+  block_gemm_f64<false>(M, CAP, Tbuf, ldt, Out, CAP, np, B, B);
+}
+
+// Block subspace iteration with Rayleigh-Ritz for n > kJacLarge.  B is the
+// block: 64 keeps H, W and T in LDS; 128 keeps H in LDS and W, T in the
+// workspace.  All products are FP64 MFMA on zero-padded np-row blocks.
+template <int B>
 __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a) {
-  constexpr int LD = CAP;
-  __shared__ double H[LD * LD];
-  __shared__ double Wl[CAP == 64 ? LD * LD : 1];
-  __shared__ double cs[LD + 2];
-  __shared__ int pr[LD + 2];
+  constexpr int LDH = B + 1, CAP = B;
+  constexpr bool SMALL = B == 64;
+  __shared__ double H[B * LDH];  // Ritz matrix; also S of the orthonormalisation; then sorted rotations
+  __shared__ double Wl[SMALL ? B * LDH : 1];
+  __shared__ double Tl[SMALL ? B * LDH : 1];
+  __shared__ double cs[B + 2];
+  __shared__ int pr[B + 2];
   __shared__ double scratch[64];
-  __shared__ double resid[LD];
-  __shared__ int order[LD];
-  __shared__ double theta[LD];
-  __shared__ int flag;
-  const int n = a.n, k = a.k, b = a.b, tid = threadIdx.x, nt = blockDim.x;
-  double* W = (CAP == 64) ? Wl : a.W;
+  __shared__ double resid[B];
+  __shared__ int order[B];
+  __shared__ double theta[B];
+  __shared__ double pbuf[6 * B];
+  __shared__ double dpiv[B];
+  constexpr int LDW = SMALL ? LDH : CAP;
+  double* W = SMALL ? Wl : a.W;
+  double* T = SMALL ? Tl : a.T;
+  const int ldt = SMALL ? LDH : CAP;
+  const int n = a.n, np = a.np, k = a.k, tid = threadIdx.x, nt = blockDim.x;
   double* V = a.V;
   double* Z = a.Z;
   double* Y = a.Y;
   double* U = a.U;
-  // V0: warm start columns, then deterministic pseudo-random columns
-  for (int e = tid; e < n * b; e += nt) {
-    const int i = e / b, j = e % b;
-    V[i * LD + j] = (a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j);
+  // zero-padded copy of G; V0: warm start columns, then pseudo-random ones
+  for (int e = tid; e < np * np; e += nt) {
+    const int i = e / np, j = e % np;
+    a.Gp[e] = (i < n && j < n) ? 0.5 * (a.G[i * n + j] + a.G[j * n + i]) : 0.0;
+  }
+  for (int e = tid; e < np * B; e += nt) {
+    const int i = e / B, j = e % B;
+    Y[i * CAP + j] = i >= n ? 0.0 : ((a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j));
   }
   __syncthreads();
-  block_orth<LD>(V, n, b, H, scratch, &flag);
+  block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch);
+  block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch);
+  { double* t = V; V = Y; Y = t; }
   int it = 0;
   for (; it < a.max_iter; ++it) {
-    // Z = G V
-    for (int e = tid; e < n * b; e += nt) {
-      const int i = e / b, j = e % b;
-      const double* g = a.G + (int64_t)i * n;
-      double s = 0.0;
-      for (int l = 0; l < n; ++l) s = fma(g[l], V[l * LD + j], s);
-      Z[i * LD + j] = s;
+    block_gemm_f64<false>(a.Gp, np, V, CAP, Z, CAP, np, B, np);  // Z = G V
+    block_gemm_f64<true>(V, CAP, Z, CAP, H, LDH, B, B, np);      // H = V^T Z
+    for (int e = tid; e < B * B; e += nt) {
+      const int i = e / B, j = e % B;
+      if (i < j) H[i * LDH + j] = H[j * LDH + i] = 0.5 * (H[i * LDH + j] + H[j * LDH + i]);
+      W[i * LDW + j] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
-    // H = V^T Z (symmetrised), W = I
-    for (int e = tid; e < b * b; e += nt) {
-      const int i = e / b, j = e % b;
-      if (j < i) continue;
-      double s = 0.0, t = 0.0;
-      for (int r = 0; r < n; ++r) {
-        s = fma(V[r * LD + i], Z[r * LD + j], s);
-        t = fma(V[r * LD + j], Z[r * LD + i], t);
-      }
-      H[i * LD + j] = H[j * LD + i] = 0.5 * (s + t);
-    }
-    for (int e = tid; e < b * b; e += nt) W[(e / b) * LD + e % b] = ((e / b) == (e % b)) ? 1.0 : 0.0;
+    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch);
+    sort_desc<LDH>(H, B, order);
+    if (tid < B) theta[tid] = H[order[tid] * LDH + order[tid]];
     __syncthreads();
-    lds_jacobi<LD>(H, W, b, 30, cs, pr, scratch);
-    sort_desc<LD>(H, b, order);
-    if (tid < b) theta[tid] = H[order[tid] * LD + order[tid]];
-    __syncthreads();
-    // Y = V W (Ritz vectors, sorted by Ritz value), U = Z W = G Y
-    for (int e = tid; e < n * b; e += nt) {
-      const int i = e / b, j = e % b;
-      const int oj = order[j];
-      double y = 0.0, z = 0.0;
-      for (int l = 0; l < b; ++l) {
-        const double w = W[l * LD + oj];
-        y = fma(V[i * LD + l], w, y);
-        z = fma(Z[i * LD + l], w, z);
-      }
-      Y[i * LD + j] = y;
-      U[i * LD + j] = z;
+    for (int e = tid; e < B * B; e += nt) {  // rotations in Ritz-value order
+      const int l = e / B, j = e % B;
+      H[l * LDH + j] = W[l * LDW + order[j]];
     }
     __syncthreads();
-    // residual norms of the leading k pairs
-    for (int r = tid; r < k; r += nt) {
+    block_gemm_f64<false>(V, CAP, H, LDH, Y, CAP, np, B, B);  // Y = V W (Ritz vectors)
+    block_gemm_f64<false>(Z, CAP, H, LDH, U, CAP, np, B, B);  // U = Z W = G Y
+    // residual norms of the leading k pairs (one wave per column)
+    for (int r = tid >> 6; r < k; r += nt >> 6) {
       double r2 = 0.0;
-      for (int i = 0; i < n; ++i) {
-        const double d = U[i * LD + r] - theta[r] * Y[i * LD + r];
+      for (int i = tid & 63; i < n; i += 64) {
+        const double d = U[i * CAP + r] - theta[r] * Y[i * CAP + r];
         r2 += d * d;
       }
-      resid[r] = sqrt(r2);
+      r2 = wave_sum(r2);
+      if ((tid & 63) == 0) resid[r] = sqrt(r2);
     }
     __syncthreads();
     if (tid == 0) {
@@ -330,14 +375,14 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     const bool done = scratch[63] != 0.0;
     __syncthreads();
     if (done || it + 1 == a.max_iter) break;
-    // V <- orth(Z W)
-    for (int e = tid; e < n * b; e += nt) V[(e / b) * LD + e % b] = U[(e / b) * LD + e % b];
-    __syncthreads();
-    block_orth<LD>(V, n, b, H, scratch, &flag);
+    // V <- orth(U), two Cholesky-QR passes (the columns of U = G Y scale
+    // with the Ritz values)
+    block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch);
+    block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch);
   }
   for (int e = tid; e < n * k; e += nt) {
     const int i = e / k, r = e % k;
-    a.evecs[i * k + r] = Y[i * LD + r];
+    a.evecs[i * k + r] = Y[i * CAP + r];
   }
   for (int r = tid; r < k; r += nt) a.evals[r] = theta[r];
   if (tid == 0 && a.iters) *a.iters = it + 1;
@@ -455,10 +500,15 @@ extern "C" int tla_mode_gram64(int dtype, const void* X, int ndim, const int64_t
   return TLA_OK;
 }
 
+static int64_t eig_np(int n) { return (int64_t)(n + 15) / 16 * 16; }
+
 extern "C" size_t tla_sym_eig_workspace_size(int n) {
   if (n <= kJac) return 0;
-  // four n x 128 blocks and the 128 x 128 rotations (sized for either block)
-  return 4 * align256(size_t(n) * kEigCapLarge * 8) + align256(size_t(kEigCapLarge) * kEigCapLarge * 8);
+  const size_t rot = align256(size_t(kEigCapLarge) * kEigCapLarge * 8);
+  if (n <= kJacLarge) return rot;  // the rotations of the Jacobi on the whole matrix
+  // padded G, four np x 128 blocks, rotations and the orthonormalising factor
+  const size_t np = (size_t)eig_np(n);
+  return align256(np * np * 8) + 4 * align256(np * kEigCapLarge * 8) + 2 * rot;
 }
 
 extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs,
@@ -468,31 +518,44 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
   TLA_REQUIRE(G && evals && evecs, "sym_eig: NULL pointer");
   TLA_REQUIRE(!Q0 || k0 >= 1, "sym_eig: warm start with k0 = %d columns", k0);
   TLA_REQUIRE(ws_bytes >= tla_sym_eig_workspace_size(n), "sym_eig: workspace too small");
+  TLA_REQUIRE(n <= kJacLarge || ws, "sym_eig: NULL workspace");
   hipStream_t st = (hipStream_t)stream;
   EigArgs a{};
   a.G = G; a.n = n; a.k = k; a.Q0 = Q0; a.q0_ld = k0;
   a.evals = evals; a.evecs = evecs; a.tol = tol > 0 ? tol : 1e-11; a.max_iter = max_iter > 0 ? max_iter : 200;
   a.iters = iters_out;
   if (n <= kJac) {
-    sym_eig_small_kernel<<<1, kEigThreads, 0, st>>>(a);
+    sym_eig_small_kernel<kJac><<<1, kEigThreads, 0, st>>>(a);
+    TLA_LAUNCH_CHECK();
+    return TLA_OK;
+  }
+  if (n <= kJacLarge) {
+    a.W = (double*)ws;
+    sym_eig_small_kernel<kJacLarge><<<1, kEigThreads, 0, st>>>(a);
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
   const bool small_block = k <= 56;
-  if (!small_block && !(k <= kEigCapLarge - 8 || n <= kEigCapLarge)) {
+  if (!small_block && k > kEigCapLarge - 8) {
     set_error("sym_eig: k = %d > %d leading eigenpairs of an n = %d > %d matrix is not supported", k,
-              kEigCapLarge - 8, n, kEigCapLarge);
+              kEigCapLarge - 8, n, kJacLarge);
     return TLA_ERR_UNSUPPORTED;
   }
-  const int cap = small_block ? 64 : kEigCapLarge;
-  const size_t blk = align256(size_t(n) * cap * 8);
-  a.b = small_block ? 64 : std::min(n, kEigCapLarge);
-  a.k0 = Q0 ? std::min(k0, a.b) : 0;
-  a.V = (double*)ws;
-  a.Z = (double*)((char*)ws + blk);
-  a.Y = (double*)((char*)ws + 2 * blk);
-  a.U = (double*)((char*)ws + 3 * blk);
-  a.W = (double*)((char*)ws + 4 * blk);
+  const int b = small_block ? 64 : kEigCapLarge;
+  a.np = (int)eig_np(n);
+  const size_t np = (size_t)a.np;
+  const size_t blk = align256(np * kEigCapLarge * 8);
+  const size_t rot = align256(size_t(kEigCapLarge) * kEigCapLarge * 8);
+  char* base = (char*)ws;
+  a.Gp = (double*)base;
+  base += align256(np * np * 8);
+  a.V = (double*)base;
+  a.Z = (double*)(base + blk);
+  a.Y = (double*)(base + 2 * blk);
+  a.U = (double*)(base + 3 * blk);
+  a.W = (double*)(base + 4 * blk);
+  a.T = (double*)(base + 4 * blk + rot);
+  a.k0 = Q0 ? std::min(k0, b) : 0;
   if (small_block)
     sym_eig_subspace_kernel<64><<<1, kEigThreads, 0, st>>>(a);
   else

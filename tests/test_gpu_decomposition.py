@@ -69,6 +69,25 @@ def test_parafac_persistent_kernels_vs_oracle(cuda, shape, prec, dimtree, rank):
     assert np.max(d) < 1e-6, (prec, dimtree, d)
 
 
+@pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("bf16x3", True)])
+def test_parafac_normalize_persistent_kernels(cuda, prec, dimtree):
+    """normalize_factors=True (weights change every sweep) with the side
+    workgroup forming V^-1 from the renormalised Grams and weights: errors
+    and weights against the FP64 oracle."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(21)
+    shape, rank = (64, 48, 128), 6
+    f_true = [rng.random((s, rank)) for s in shape]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(rank), f_true, dtype=np.float32), 0.01, rng)
+    (w_ref, _), ref_errs = T.parafac(x.astype(np.float64), rank, n_iter_max=10, init="random", random_state=8,
+                                     tol=0, normalize_factors=True, return_errors=True)
+    cp, errs = tla.parafac(g(x, cuda, torch.float32), rank, n_iter_max=10, init="random", random_state=8, tol=0,
+                           normalize_factors=True, return_errors=True, precision=prec, dimtree=dimtree)
+    d = np.abs(np.array(errs) ** 2 - np.array(ref_errs) ** 2)
+    assert np.max(d) < 1e-6, d
+    np.testing.assert_allclose(cp.weights.cpu().numpy(), w_ref, rtol=1e-3)
+
+
 def test_parafac_order4_dimtree(cuda):
     import tensorly_amd as tla
     rng = np.random.default_rng(5)
