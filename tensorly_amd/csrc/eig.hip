@@ -126,30 +126,32 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         cs[2 * a + 1] = s;
       }
       __syncthreads();
-      // rows: A[p,:], A[q,:]
+      // A <- J^T A J and W <- W J in one pass: each thread owns whole 2 x 2
+      // blocks (rows of pair pa, columns of pair pb) and whole (row, pair)
+      // pieces of W, so no entry is touched twice
       const int npairs = me / 2;
-      for (int e = tid; e < npairs * m; e += nt) {
-        const int a = e / m, j = e % m;
-        const int p = pr[2 * a], q = pr[2 * a + 1];
-        if (q < m) {
-          const double c = cs[2 * a], s = cs[2 * a + 1];
-          const double x = A[p * LDA + j], y = A[q * LDA + j];
-          A[p * LDA + j] = c * x - s * y;
-          A[q * LDA + j] = s * x + c * y;
-        }
+      for (int e = tid; e < npairs * npairs; e += nt) {
+        const int pa = e / npairs, pb = e % npairs;
+        const int p0 = pr[2 * pa], q0 = pr[2 * pa + 1], p1 = pr[2 * pb], q1 = pr[2 * pb + 1];
+        const double ca = cs[2 * pa], sa = cs[2 * pa + 1], cb = cs[2 * pb], sb = cs[2 * pb + 1];
+        const bool r1 = q0 < m, c1 = q1 < m;  // the dummy slot has no row / column
+        const double a00 = A[p0 * LDA + p1];
+        const double a01 = c1 ? A[p0 * LDA + q1] : 0.0;
+        const double a10 = r1 ? A[q0 * LDA + p1] : 0.0;
+        const double a11 = (r1 && c1) ? A[q0 * LDA + q1] : 0.0;
+        const double u00 = ca * a00 - sa * a10, u01 = ca * a01 - sa * a11;
+        const double u10 = sa * a00 + ca * a10, u11 = sa * a01 + ca * a11;
+        A[p0 * LDA + p1] = cb * u00 - sb * u01;
+        if (c1) A[p0 * LDA + q1] = sb * u00 + cb * u01;
+        if (r1) A[q0 * LDA + p1] = cb * u10 - sb * u11;
+        if (r1 && c1) A[q0 * LDA + q1] = sb * u10 + cb * u11;
       }
-      __syncthreads();
-      // columns of A and W
       for (int e = tid; e < npairs * m; e += nt) {
-        const int a = e % npairs, i = e / npairs;
-        const int p = pr[2 * a], q = pr[2 * a + 1];
+        const int pb = e % npairs, i = e / npairs;
+        const int p = pr[2 * pb], q = pr[2 * pb + 1];
         if (q < m) {
-          const double c = cs[2 * a], s = cs[2 * a + 1];
-          double x = A[i * LDA + p], y = A[i * LDA + q];
-          A[i * LDA + p] = c * x - s * y;
-          A[i * LDA + q] = s * x + c * y;
-          x = W[i * LDW + p];
-          y = W[i * LDW + q];
+          const double c = cs[2 * pb], s = cs[2 * pb + 1];
+          const double x = W[i * LDW + p], y = W[i * LDW + q];
           W[i * LDW + p] = c * x - s * y;
           W[i * LDW + q] = s * x + c * y;
         }
