@@ -547,15 +547,28 @@ __device__ __forceinline__ void scale_q(const StreamArgs& a, int64_t q, int c, f
   }
 }
 
-template <typename XT>
-__device__ __forceinline__ void load_pair(const XT* p, floatx8& v0, floatx8& v1) {
+// 8 consecutive tensor values at p (16-byte aligned); NT: non-temporal
+// (streaming) loads, which measured faster for the kernels whose output is
+// small, and slower for the T01 partial that writes 33.5 MB while it reads.
+template <typename XT, bool NT>
+__device__ __forceinline__ floatx8 load_stream_x8(const XT* p) {
   if constexpr (sizeof(XT) == 2) {
-    v0 = __builtin_convertvector(*(const bf16x8*)p, floatx8);
-    v1 = __builtin_convertvector(*(const bf16x8*)(p + 32), floatx8);
+    const bf16x8 v = NT ? __builtin_nontemporal_load((const bf16x8*)p) : *(const bf16x8*)p;
+    return __builtin_convertvector(v, floatx8);
   } else {
-    v0 = load_x8<XT>(p);
-    v1 = load_x8<XT>(p + 32);
+    const floatx4 lo = NT ? __builtin_nontemporal_load((const floatx4*)p) : *(const floatx4*)p;
+    const floatx4 hi = NT ? __builtin_nontemporal_load((const floatx4*)(p + 4)) : *(const floatx4*)(p + 4);
+    floatx8 v;
+    v[0] = lo[0]; v[1] = lo[1]; v[2] = lo[2]; v[3] = lo[3];
+    v[4] = hi[0]; v[5] = hi[1]; v[6] = hi[2]; v[7] = hi[3];
+    return v;
   }
+}
+
+template <typename XT, bool NT = false>
+__device__ __forceinline__ void load_pair(const XT* p, floatx8& v0, floatx8& v1) {
+  v0 = load_stream_x8<XT, NT>(p);
+  v1 = load_stream_x8<XT, NT>(p + 32);
 }
 
 template <int PREC, int RT>
@@ -679,7 +692,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   int64_t ld_tile = w;
   int ld_kk = 0;
   auto issue = [&](StreamSlot<RT>& sl) {
-    load_pair<XT>(X + (ld_tile * 16 + c) * args.K + ld_kk + 8 * g, sl.a, sl.b);
+    load_pair<XT, EPI != SE_ROWS>(X + (ld_tile * 16 + c) * args.K + ld_kk + 8 * g, sl.a, sl.b);
     if (ld_kk + 64 == K) {
       if constexpr (EPI != SE_ROWS) {
         const int64_t q = ld_tile / tpq;
@@ -798,6 +811,12 @@ __device__ __forceinline__ floatx4 load_x4<float>(const float* p) {
 template <>
 __device__ __forceinline__ floatx4 load_x4<__bf16>(const __bf16* p) {
   return __builtin_convertvector(*(const bf16x4*)p, floatx4);
+}
+// the same with non-temporal (streaming) loads
+template <typename XT>
+__device__ __forceinline__ floatx4 load_x4_nt(const XT* p) {
+  if constexpr (sizeof(XT) == 2) return __builtin_convertvector(__builtin_nontemporal_load((const bf16x4*)p), floatx4);
+  else return __builtin_nontemporal_load((const floatx4*)p);
 }
 
 template <int PREC, typename XT, int RT>
@@ -952,7 +971,7 @@ __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* 
   for (int u = 0; u < 8; ++u) {
     const int64_t row = rowb + u;
     sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-    if (row < pe && colok) sl.x[u] = load_x4<XT>(X + row * K + col);
+    if (row < pe && colok) sl.x[u] = load_x4_nt<XT>(X + row * K + col);
   }
   // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0
   const bool okrow = rowb < pe;
@@ -1028,7 +1047,7 @@ __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* 
   for (int u = 0; u < 8; ++u) {
     const int64_t row = rowb + u;
     sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-    if (row < pe && colok) sl.x[u] = load_x4<XT>(X + row * K + col);
+    if (row < pe && colok) sl.x[u] = load_x4_nt<XT>(X + row * K + col);
   }
   // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0; a row
   // block past the end reads F_j row 0 (its x and gm are zero)
