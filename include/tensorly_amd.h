@@ -124,10 +124,10 @@ int tla_gram64(int dtype, const void* A, int64_t rows, int rank, double* G, void
 /* ---------------------------------------------------------------------
  * ALS factor update (tensorly/decomposition/_cp.py:411-427):
  *   V = (prod_{i != mode} G_i + l2_reg I) * (w w^T)          (FP64)
- *   factor = mttkrp @ inv(V)          (FP64 SPD elimination, i.e. the
- *                                      inverse from the LDL^T / Cholesky
- *                                      pivots; LU with partial pivoting if
- *                                      V is not positive definite)
+ *   factor = mttkrp @ inv(V)          (FP64 Gauss-Jordan elimination
+ *                                      without pivoting, valid for SPD V;
+ *                                      partial pivoting if a pivot is not
+ *                                      positive; products on FP64 MFMA)
  * and, fused, the FP64 Gram of the new factor (gram_out) and, if
  * iprod_out != NULL, <mttkrp, factor> (the error_calc inner product,
  * _cp.py:222).  grams: host array of ndim device pointers to R x R FP64.
@@ -171,11 +171,14 @@ int tla_cp_error(const double* const* grams, int ndim, const void* weights, int 
  *   tla_mode_gram64: G = unfold(X, mode) unfold(X, mode)^T (I_mode^2 FP64),
  *                    without forming the unfolding.
  *   tla_sym_eig:     k leading eigenpairs (descending) of symmetric FP64 G
- *                    (n x n).  n <= 64: Jacobi; n > 64: subspace iteration
- *                    with Rayleigh-Ritz, block 64, k <= 56, optional warm
- *                    start Q0 (n x k0), converged when every leading residual
- *                    is <= tol * lambda_1 (or after max_iter steps; the step
- *                    count goes to *iters_out if non-NULL, device int).
+ *                    (n x n).  n <= 128: Jacobi on the whole matrix (any
+ *                    k <= n); n > 128: subspace iteration with Rayleigh-Ritz,
+ *                    block 64 for k <= 56 or 128 for k <= 120 (larger k
+ *                    returns TLA_ERR_UNSUPPORTED), optional warm start Q0
+ *                    (n x k0, its first columns used), converged when every
+ *                    leading residual is <= tol * lambda_1 (or after
+ *                    max_iter steps; the sweep or step count goes to
+ *                    *iters_out if non-NULL, device int).
  * ------------------------------------------------------------------- */
 size_t tla_mode_gram64_workspace_size(int ndim, const int64_t* shape, int mode);
 int tla_mode_gram64(int dtype, const void* X, int ndim, const int64_t* shape, int mode, double* G,
