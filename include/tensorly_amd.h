@@ -173,7 +173,7 @@ int tla_cp_error(const double* const* grams, int ndim, const void* weights, int 
  *   tla_sym_eig:     k leading eigenpairs (descending) of symmetric FP64 G
  *                    (n x n).  n <= 128: Jacobi on the whole matrix (any
  *                    k <= n); n > 128: subspace iteration with Rayleigh-Ritz,
- *                    block 64 for k <= 56 or 128 for k <= 120 (larger k
+ *                    block 48 / 64 / 128 for k <= 40 / 56 / 120 (larger k
  *                    returns TLA_ERR_UNSUPPORTED), optional warm start Q0
  *                    (n x k0, its first columns used), converged when every
  *                    leading residual is <= tol * lambda_1 (or after

@@ -10,8 +10,8 @@
 //    ordering, all n/2 rotations of a round applied in parallel); the matrix
 //    in LDS with an odd leading dimension, the rotations in LDS (n <= 64) or
 //    in the workspace.
-//  * n  > 128: block subspace iteration with Rayleigh-Ritz, block b = 64 for
-//    k <= 56 or 128 for k <= 120.  Each step: Z = G V, H = V^T Z on FP64
+//  * n  > 128: block subspace iteration with Rayleigh-Ritz, block b = 48 for
+//    k <= 40, 64 for k <= 56 or 128 for k <= 120.  Each step: Z = G V, H = V^T Z on FP64
 //    MFMA, Jacobi(H) = W diag(t) W^T (warm: the previous Ritz basis nearly
 //    diagonalises H), Ritz vectors Y = V W and U = Z W = G Y, residuals of
 //    the k leading pairs, V <- orth(U) by two shifted Cholesky-QR passes
@@ -75,6 +75,20 @@ __device__ void block_gemm_f64(const double* A, int lda, const double* B, int ld
 // rotations into W (leading dimension LDW, in LDS or global memory,
 // initialised by the caller).  Returns the number of sweeps.
 // ---------------------------------------------------------------------------
+// 1 / x and 1 / sqrt(x) for finite x > 0 (here x > 1e-300): the hardware
+// estimates refined by two Newton steps each, to full double precision.
+__device__ __forceinline__ double jac_recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double jac_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  return y * fma(-hx * y, y, 1.5);
+}
+
 template <int LDA, int LDW>
 __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch) {
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -113,10 +127,22 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
         if (q < m) {
           const double apq = A[p * LDA + q];
           if (fabs(apq) > 1e-300) {
+            // tan of the rotation angle t = sgn(tau) / (|tau| + sqrt(1 + tau^2)),
+            // c = 1 / sqrt(1 + t^2); reciprocals and reciprocal square roots
+            // from the hardware estimates refined by Newton steps (this
+            // is on the critical path of every round)
             const double app = A[p * LDA + p], aqq = A[q * LDA + q];
-            const double tau = (aqq - app) / (2.0 * apq);
-            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
+            const double tau = (aqq - app) * jac_recip(2.0 * apq);
+            const double at = fabs(tau);
+            double t;
+            if (at > 1e100) {
+              t = 0.5 / at;
+            } else {
+              const double u = 1.0 + tau * tau;
+              t = jac_recip(at + u * jac_rsqrt(u));
+            }
+            t = tau >= 0.0 ? t : -t;
+            c = jac_rsqrt(1.0 + t * t);
             s = t * c;
           }
         }
@@ -244,7 +270,9 @@ template <int B>
 __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, double* Tbuf, int ldt, bool shift,
                            double* pbuf, double* dpiv, double* scratch) {
   constexpr int LDS_S = B + 1, CAP = B;
-  constexpr int TPR = kEigThreads / B, CPT = 2 * B / TPR;
+  // TPR threads per row of [S | I] (rows past B idle), CPT columns each
+  constexpr int TPR = B > 64 ? 4 : 8, CPT = 2 * B / TPR;
+  static_assert(TPR * B <= kEigThreads && CPT * TPR == 2 * B && TPR % 2 == 0, "thread layout");
   block_gemm_f64<true>(M, CAP, M, CAP, Sbuf, LDS_S, B, B, np);
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -257,11 +285,12 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
   const double sh = shift ? 1e-13 * mx * (double)np : 0.0;
   const double floor = 1e-300 + 1e-30 * mx;
   const int i = tid / TPR, seg = tid % TPR, c0 = seg * CPT;
+  const bool live = i < B;
   double x[CPT];
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int c = c0 + j;
-    x[j] = c < B ? Sbuf[i * LDS_S + c] + (c == i ? sh : 0.0) : ((c - B == i) ? 1.0 : 0.0);
+    x[j] = !live ? 0.0 : (c < B ? Sbuf[i * LDS_S + c] + (c == i ? sh : 0.0) : ((c - B == i) ? 1.0 : 0.0));
   }
   for (int kb = 0; kb < TPR / 2; ++kb) {
 #pragma unroll
@@ -273,11 +302,11 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
 #pragma unroll
         for (int j = 0; j < CPT; ++j) row[c0 + j] = x[j];
       }
-      if (seg == kb) col[i] = x[kk];
+      if (live && seg == kb) col[i] = x[kk];
       __syncthreads();
       const double p = fmax(col[k], floor);
       if (i == k && seg == 0) dpiv[k] = p;
-      if (i > k) {
+      if (live && i > k) {
         const double f = col[i] / p;
 #pragma unroll
         for (int j = 0; j < CPT; ++j) x[j] = fma(-f, row[c0 + j], x[j]);
@@ -286,23 +315,23 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
   }
   __syncthreads();
   // x now holds row i of [D L^T | L^-1]; T[r][i] = L^-1[i][r] / sqrt(D[i])
-  const double ds = 1.0 / sqrt(dpiv[i]);
+  const double ds = live ? 1.0 / sqrt(dpiv[i]) : 0.0;
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int c = c0 + j;
-    if (c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds;
+    if (live && c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds;
   }
   __syncthreads();
   block_gemm_f64<false>(M, CAP, Tbuf, ldt, Out, CAP, np, B, B);
 }
 
 // Block subspace iteration with Rayleigh-Ritz for n > kJacLarge.  B is the
-// block: 64 keeps H, W and T in LDS; 128 keeps H in LDS and W, T in the
+// block: 48 and 64 keep H, W and T in LDS; 128 keeps H in LDS and W, T in the
 // workspace.  All products are FP64 MFMA on zero-padded np-row blocks.
 template <int B>
 __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a) {
   constexpr int LDH = B + 1, CAP = B;
-  constexpr bool SMALL = B == 64;
+  constexpr bool SMALL = B <= 64;
   __shared__ double H[B * LDH];  // Ritz matrix; also S of the orthonormalisation; then sorted rotations
   __shared__ double Wl[SMALL ? B * LDH : 1];
   __shared__ double Tl[SMALL ? B * LDH : 1];
@@ -537,13 +566,14 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
+  // block: 48 for k <= 40 (cheaper Jacobi sweeps), 64 for k <= 56, 128 for k <= 120
   const bool small_block = k <= 56;
   if (!small_block && k > kEigCapLarge - 8) {
     set_error("sym_eig: k = %d > %d leading eigenpairs of an n = %d > %d matrix is not supported", k,
               kEigCapLarge - 8, n, kJacLarge);
     return TLA_ERR_UNSUPPORTED;
   }
-  const int b = small_block ? 64 : kEigCapLarge;
+  const int b = k <= 40 ? 48 : (small_block ? 64 : kEigCapLarge);
   a.np = (int)eig_np(n);
   const size_t np = (size_t)a.np;
   const size_t blk = align256(np * kEigCapLarge * 8);
@@ -558,7 +588,9 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
   a.W = (double*)(base + 4 * blk);
   a.T = (double*)(base + 4 * blk + rot);
   a.k0 = Q0 ? std::min(k0, b) : 0;
-  if (small_block)
+  if (b == 48)
+    sym_eig_subspace_kernel<48><<<1, kEigThreads, 0, st>>>(a);
+  else if (small_block)
     sym_eig_subspace_kernel<64><<<1, kEigThreads, 0, st>>>(a);
   else
     sym_eig_subspace_kernel<kEigCapLarge><<<1, kEigThreads, 0, st>>>(a);

@@ -184,8 +184,8 @@ def test_parafac_fixed_modes_l2_callback(cuda):
     assert len(seen) == 4
 
 
-@pytest.mark.parametrize("n,k", [(40, 5), (64, 64), (200, 32), (256, 56), (100, 57), (128, 64), (128, 128),
-                                 (200, 100), (300, 120)])
+@pytest.mark.parametrize("n,k", [(40, 5), (64, 64), (200, 32), (256, 40), (300, 41), (256, 56), (100, 57),
+                                 (128, 64), (128, 128), (200, 100), (300, 120)])
 def test_gram_eigensolver(cuda, n, k):
     from tensorly_amd.svd import gram_eigh
     rng = np.random.default_rng(n)
