@@ -90,7 +90,8 @@ __device__ __forceinline__ double jac_rsqrt(double x) {
 }
 
 template <int LDA, int LDW>
-__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch) {
+__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch,
+                          double rel_off2 = 1e-30) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int me = m + (m & 1);  // even number of slots (one dummy)
   int sweep = 0;
@@ -114,7 +115,7 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
       scratch[33] = t;
     }
     __syncthreads();
-    if (!(scratch[32] > 1e-30 * scratch[33]) || scratch[33] == 0.0) break;
+    if (!(scratch[32] > rel_off2 * scratch[33]) || scratch[33] == 0.0) break;
     for (int round = 0; round < me - 1; ++round) {
       // round-robin pairing: slot 0 fixed, others rotate
       if (tid < me / 2) {
@@ -365,6 +366,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
   block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch);
   block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch);
   { double* t = V; V = Y; Y = t; }
+  const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
+  double jthr = 1e-10;
   int it = 0;
   for (; it < a.max_iter; ++it) {
     block_gemm_f64<false>(a.Gp, np, V, CAP, Z, CAP, np, B, np);  // Z = G V
@@ -375,7 +378,10 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
       W[i * LDW + j] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
-    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch);
+    // the Ritz rotations only need the accuracy of the current subspace:
+    // off-diagonal mass down to (residual / t_1)^2 / 100 of the total,
+    // tightening to tol^2 / 100 as the residuals fall
+    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch, jthr);
     sort_desc<LDH>(H, B, order);
     if (tid < B) theta[tid] = H[order[tid] * LDH + order[tid]];
     __syncthreads();
@@ -400,10 +406,14 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     if (tid == 0) {
       double worst = 0.0;
       for (int r = 0; r < k; ++r) worst = fmax(worst, resid[r]);
-      scratch[63] = (worst <= a.tol * fmax(fabs(theta[0]), 1e-300)) ? 1.0 : 0.0;
+      const double t1 = fmax(fabs(theta[0]), 1e-300);
+      scratch[63] = (worst <= a.tol * t1) ? 1.0 : 0.0;
+      const double rr = worst / t1;
+      scratch[62] = fmin(1e-10, fmax(jfloor, 1e-2 * rr * rr));
     }
     __syncthreads();
     const bool done = scratch[63] != 0.0;
+    jthr = scratch[62];
     __syncthreads();
     if (done || it + 1 == a.max_iter) break;
     // V <- orth(U), two Cholesky-QR passes (the columns of U = G Y scale
