@@ -399,12 +399,14 @@ def batched_report(torch, dev, net, log):
     f0 = b.random_init(seed=0)
     max_sweeps = 1000
     res = {"problems": len(probs), "tensor_bytes": int(b.X.numel() * 4), "tol": 1e-7, "max_sweeps": max_sweeps,
-           "autotune": {"chosen": b.config, "table": table}}
+           "autotune_throughput": {"chosen": b.config, "table": table}}
     policies = [("plain", {}), ("linesearch", {})]
     if net is not None:
         policies.append(("learned", {"model": net, "alpha_max": 16.0}))
     hist = {}
     for name, kw in policies:
+        # launch shape tuned on whole runs of this policy from another random start (seed 1)
+        tt = b.autotune(converge=True, policy=name, seed=1, repeats=1, max_sweeps=max_sweeps, tol=1e-7, **kw)
         out = b.make_outputs(max_sweeps, history=True)
         f = f0.clone()
         b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)  # warm-up
@@ -428,7 +430,9 @@ def batched_report(torch, dev, net, log):
                      "tensor_GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
                      "failed": int((out["status"].cpu().numpy() != 0).sum()),
                      "mean_final_error": float(out["error"].cpu().numpy().mean()),
-                     "extrapolations": ne, "fallback_rate": (1.0 - na / ne) if ne else None}
+                     "extrapolations": ne, "fallback_rate": (1.0 - na / ne) if ne else None,
+                     "launch": dict(b.config), "autotune_ms": {f"{t['waves']}x{t['blocks_per_cu']}": round(t["ms"], 1)
+                                                               for t in tt}}
         sw_all = out["sweeps"].cpu().numpy()
         hist[name] = (out["hist_err"].cpu().numpy(), sw_all, out["error"].cpu().numpy())
         log(f"[bench] batched {name}: {res[name]}")

@@ -221,10 +221,18 @@ class BatchedCP:
             self.launch(factors, out, **kw)
         return g
 
-    def autotune(self, candidates=None, sweeps=6, seed=0, repeats=2):
+    def autotune(self, candidates=None, sweeps=6, seed=0, repeats=2, converge=False, policy="plain",
+                 max_sweeps=1000, tol=1e-7, **policy_kw):
         """Time candidate launch shapes (waves per workgroup x resident
-        workgroups per CU) on ``sweeps`` plain sweeps of the whole batch and
-        keep the fastest.  Returns the timing table."""
+        workgroups per CU) and keep the fastest.  Returns the timing table.
+
+        ``converge=False`` times ``sweeps`` plain sweeps of every problem: a
+        throughput measure.  ``converge=True`` times whole runs of ``policy``
+        to ``tol`` (at most ``max_sweeps``): the launch then lasts as long as
+        its slowest problems, whose sweep counts vary 3-5x across a batch, and
+        wider workgroups finish those sooner at some cost in throughput.
+        Start from the factors of ``random_init(seed)``, so that a caller
+        can tune on other starting points than the ones it times."""
         if candidates is None:
             candidates = []
             for w in (2, 4, 8):
@@ -232,12 +240,16 @@ class BatchedCP:
                 for b in sorted({per_cu, max(1, per_cu // 2)}):
                     candidates.append((w, b))
         f0 = self.random_init(seed)
-        out = self.make_outputs(sweeps)
+        if converge:
+            pol, n_sw, tl, kw = policy, int(max_sweeps), float(tol), dict(policy_kw)
+        else:
+            pol, n_sw, tl, kw = "plain", int(sweeps), 0.0, {}
+        out = self.make_outputs(n_sw, history=pol in ("learned", "probe"))
         table = []
         for w, b in candidates:
             grid = self._grid(w, b)
             f = f0.clone()
-            self.launch(f, out, policy="plain", max_sweeps=sweeps, tol=0.0, waves=w, grid=grid)
+            self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, **kw)
             torch.cuda.synchronize(self.device)
             best = float("inf")
             for _ in range(repeats):
@@ -245,7 +257,7 @@ class BatchedCP:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                self.launch(f, out, policy="plain", max_sweeps=sweeps, tol=0.0, waves=w, grid=grid)
+                self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, **kw)
                 e1.record()
                 torch.cuda.synchronize(self.device)
                 best = min(best, e0.elapsed_time(e1))

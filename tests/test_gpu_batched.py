@@ -263,3 +263,17 @@ def test_autotune_picks_a_candidate(cuda):
         f_b, o_b = b.run(f0, policy="plain", max_sweeps=2, tol=0.0, waves=w, grid=b._grid(w))
         # the reduction split over waves changes the FP32 summation order only
         _assert_err_close(o_b["error"].cpu().numpy(), o_a["error"].cpu().numpy())
+
+
+def test_autotune_on_converging_runs(cuda):
+    probs = _problems(10, 48, size_range=(8, 40), rank_range=(2, 8), congruence=0.5)
+    b = _batch(cuda, probs)
+    table = b.autotune(converge=True, policy="learned", seed=1, repeats=1, max_sweeps=40, tol=1e-6,
+                       alpha_fixed=1.0)
+    assert len(table) >= 3 and all(t["ms"] > 0 for t in table)
+    win = min(table, key=lambda t: t["ms"])
+    assert b.config == {"waves": win["waves"], "blocks_per_cu": win["blocks_per_cu"]}
+    # the tuned shape is used by later launches and solves the batch like any other shape
+    _, o = b.run(b.random_init(0), policy="plain", max_sweeps=40, tol=1e-6)
+    assert (o["status"].cpu().numpy() == 0).all()
+    assert (o["sweeps"].cpu().numpy() >= 1).all()
