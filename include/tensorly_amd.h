@@ -162,6 +162,21 @@ int tla_cp_error(const double* const* grams, int ndim, const void* weights, int 
                  const double* normx2, const double* iprod, double* errors, int slot,
                  void* stream);
 
+/* ---------------------------------------------------------------------
+ * Multiplicative update of one factor of a non-negative CP model
+ * (tensorly/decomposition/_nn_cp.py:111-135, the mode loop of
+ * non_negative_parafac):
+ *   accum  = (prod_{m != mode} G_m) * w w^T          (FP64, R x R)
+ *   factor = factor * max(mttkrp, eps) / max(factor accum, eps)
+ * in place.  factor and mttkrp are rows x rank in `dtype` (TLA_F32 /
+ * TLA_F64); grams is a host array of ndim device pointers to FP64 R x R
+ * Grams (entry `mode` unused); weights (nullable: ones) in `wdtype`.
+ * rank <= 64.
+ * ------------------------------------------------------------------- */
+int tla_nn_mu_update(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
+                     const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
+                     double eps, void* stream);
+
 
 /* ---------------------------------------------------------------------
  * Leading left singular vectors through the FP64 Gram (replaces the LAPACK

@@ -29,6 +29,13 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
   least-squares optimality (gradient = 0 after each mode update).
 * ``svd_flip``/``truncated_svd``/``symeig_svd``: LAPACK identities
   (``U S V = A``, orthonormality) and the sign rule checked on hand-built cases.
+* ``non_negative_parafac``: the Lee-Seung monotone decrease of the error,
+  exact factorisations (with non-unit weights) as fixed points of the update,
+  and a perturbed factor moving towards the solution
+  (``tests/test_oracle_nn_cp.py``).
+* ``make_svd_non_negative``: a non-negative rank-1 matrix reproduced exactly,
+  non-negative outputs, the leading triplet kept, the NNDSVDa fill value, and
+  the dominant-sign choice on a hand-built column pair.
 """
 import math
 import warnings
@@ -41,7 +48,8 @@ __all__ = [
     "unfolding_dot_khatri_rao_einsum", "mode_dot", "multi_mode_dot",
     "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
     "cp_norm", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
-    "svd_interface", "initialize_cp", "error_calc", "parafac",
+    "svd_interface", "make_svd_non_negative", "initialize_cp", "error_calc", "parafac",
+    "non_negative_parafac",
     "validate_tucker_rank", "tucker_to_tensor", "initialize_tucker",
     "partial_tucker", "tucker",
 ]
@@ -398,10 +406,46 @@ def symeig_svd(matrix, n_eigenvecs=None, **kwargs):
     return U[:, : min(d1, n)], S[: min(d1, d2, n)], V[: min(d2, n), :]
 
 
+def make_svd_non_negative(tensor, U, S, V, nntype=True):
+    """``tenalg/svd.py:68-127``: NNDSVD (``'nndsvd'``: small entries set to
+    zero by soft thresholding at eps) or NNDSVDa (``True`` / ``'nndsvda'``:
+    small entries set to the mean of ``tensor``)."""
+    if nntype is True:
+        nntype = "nndsvda"
+    W = np.zeros_like(U)
+    H = np.zeros_like(V)
+    W[:, 0] = np.sqrt(S[0]) * np.abs(U[:, 0])
+    H[0, :] = np.sqrt(S[0]) * np.abs(V[0, :])
+    for j in range(1, min(U.shape[1], V.shape[0])):
+        x, y = U[:, j], V[j, :]
+        x_p, y_p = np.clip(x, 0.0, None), np.clip(y, 0.0, None)
+        x_n, y_n = np.abs(np.clip(x, None, 0.0)), np.abs(np.clip(y, None, 0.0))
+        x_p_nrm, y_p_nrm = np.linalg.norm(x_p), np.linalg.norm(y_p)
+        x_n_nrm, y_n_nrm = np.linalg.norm(x_n), np.linalg.norm(y_n)
+        m_p, m_n = x_p_nrm * y_p_nrm, x_n_nrm * y_n_nrm
+        if m_p > m_n:
+            u, v, sigma = x_p / x_p_nrm, y_p / y_p_nrm, m_p
+        else:
+            u, v, sigma = x_n / x_n_nrm, y_n / y_n_nrm, m_n
+        lbd = np.sqrt(S[j] * sigma)
+        W[:, j] = lbd * u
+        H[j, :] = lbd * v
+    eps = np.finfo(tensor.dtype).eps
+    if nntype == "nndsvd":
+        W = np.sign(W) * np.clip(np.abs(W) - eps, 0.0, None)
+        H = np.sign(H) * np.clip(np.abs(H) - eps, 0.0, None)
+    elif nntype == "nndsvda":
+        avg = np.mean(tensor)
+        W = np.where(W < eps, np.ones(W.shape, dtype=W.dtype) * avg, W)
+        H = np.where(H < eps, np.ones(H.shape, dtype=H.dtype) * avg, H)
+    else:
+        raise ValueError(f'Invalid nntype parameter: got {nntype} instead of one of ("nndsvd", "nndsvda")')
+    return W, H
+
+
 def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True,
-                  u_based_flip_sign=True, **kwargs):
-    """``tenalg/svd.py:366-447`` without mask imputation / non-negativity
-    (neither is on the path)."""
+                  u_based_flip_sign=True, non_negative=None, **kwargs):
+    """``tenalg/svd.py:366-447`` without mask imputation (not on the path)."""
     if method == "truncated_svd":
         fun = truncated_svd
     elif method == "symeig_svd":
@@ -414,6 +458,8 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
     U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs)
     if flip_sign:
         U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
+    if non_negative is not False and non_negative is not None:
+        U, V = make_svd_non_negative(matrix, U, S, V, non_negative)
     return U, S, V
 
 
@@ -421,8 +467,8 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
 # CP-ALS  (decomposition/_cp.py)
 # --------------------------------------------------------------------------
 def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=None,
-                  normalize_factors=False):
-    """``decomposition/_cp.py:26-137`` (no mask / non-negative branches)."""
+                  normalize_factors=False, non_negative=False):
+    """``decomposition/_cp.py:26-137`` (no mask)."""
     rng = check_random_state(random_state)
     if isinstance(init, str) and init == "random":
         weights, factors = random_cp(tensor.shape, rank, normalise_factors=False,
@@ -430,7 +476,8 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=No
     elif isinstance(init, str) and init == "svd":
         factors = []
         for mode in range(tensor.ndim):
-            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd)
+            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd,
+                                    non_negative=non_negative)
             if mode == 0:
                 k = min(rank, S.shape[0])
                 U = U.copy()
@@ -450,6 +497,8 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=No
         return np.ones(rank, dtype=factors[0].dtype), [f * avg for f in factors]
     else:
         raise ValueError(f'Initialization method "{init}" not recognized')
+    if non_negative:
+        factors = [np.abs(f) for f in factors]
     if normalize_factors:
         weights, factors = cp_normalize((weights, factors))
     return weights, factors
@@ -573,6 +622,64 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
 
 # --------------------------------------------------------------------------
 # Tucker  (tucker_tensor.py, decomposition/_tucker.py)
+# --------------------------------------------------------------------------
+# Non-negative CP by multiplicative updates  (decomposition/_nn_cp.py)
+# --------------------------------------------------------------------------
+def non_negative_parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", tol=10e-7,
+                         random_state=None, normalize_factors=False, return_errors=False,
+                         cvg_criterion="abs_rec_error", fixed_modes=None):
+    """``decomposition/_nn_cp.py:26-180`` (no mask, no verbose output).
+
+    Per mode (``:111-135``): ``accum`` = Hadamard of the other modes' Grams
+    times ``w w^T``; ``A <- A * clip(M, eps) / clip(A accum, eps)`` with
+    ``eps`` the machine epsilon of the tensor's dtype.  The error (``:138-150``)
+    comes from the last mode's MTTKRP as in CP-ALS."""
+    epsilon = np.finfo(tensor.dtype).eps
+    rank = validate_cp_rank(tensor.shape, rank=rank)
+    weights, factors = initialize_cp(tensor, rank, init=init, svd=svd, non_negative=True,
+                                     random_state=random_state, normalize_factors=normalize_factors)
+    factors = [np.array(f) for f in factors]
+    rec_errors = []
+    norm_tensor = norm(tensor, 2)
+    fixed_modes = [] if fixed_modes is None else list(fixed_modes)
+    if tensor.ndim - 1 in fixed_modes:
+        warnings.warn("You asked for fixing the last mode, which is not supported while tol is fixed.")
+        fixed_modes.remove(tensor.ndim - 1)
+    modes_list = [mode for mode in range(tensor.ndim) if mode not in fixed_modes]
+    mttkrp = None
+    for iteration in range(n_iter_max):
+        for mode in modes_list:
+            accum = None
+            for e in [i for i in range(len(factors)) if i != mode]:
+                g = np.dot(factors[e].T, factors[e])
+                accum = g if accum is None else accum * g
+            accum = weights.reshape(-1, 1) * accum * weights.reshape(1, -1)
+            mttkrp = unfolding_dot_khatri_rao(tensor, (weights, factors), mode)
+            numerator = np.clip(mttkrp, epsilon, None)
+            denominator = np.clip(np.dot(factors[mode], accum), epsilon, None)
+            factors[mode] = factors[mode] * numerator / denominator
+            if normalize_factors and mode != modes_list[-1]:
+                weights, factors = cp_normalize((weights, factors))
+        if tol:
+            rec_error = error_calc(tensor, norm_tensor, weights, factors, mttkrp=mttkrp) / norm_tensor
+            rec_errors.append(rec_error)
+            if iteration >= 1:
+                dec = rec_errors[-2] - rec_errors[-1]
+                if cvg_criterion == "abs_rec_error":
+                    stop = abs(dec) < tol
+                elif cvg_criterion == "rec_error":
+                    stop = dec < tol
+                else:
+                    raise TypeError("Unknown convergence criterion")
+                if stop:
+                    break
+        if normalize_factors:
+            weights, factors = cp_normalize((weights, factors))
+    if return_errors:
+        return (weights, factors), rec_errors
+    return weights, factors
+
+
 # --------------------------------------------------------------------------
 def validate_tucker_rank(tensor_shape, rank="same", rounding="round", fixed_modes=None):
     """``tucker_tensor.py:344-447`` (int and explicit-tuple forms, and the

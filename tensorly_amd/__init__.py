@@ -6,7 +6,8 @@ Drop-ins keep the reference signatures (tensorly 0.9.0):
 * tenalg backend functions: :func:`khatri_rao`, :func:`unfolding_dot_khatri_rao`,
   :func:`mode_dot`, :func:`multi_mode_dot` (:func:`tenalg.register` plugs them
   into tensorly's tenalg dispatcher when tensorly is installed);
-* decompositions: :func:`parafac`, :func:`tucker`, :func:`partial_tucker`.
+* decompositions: :func:`parafac`, :func:`non_negative_parafac`, :func:`tucker`,
+  :func:`partial_tucker`.
 
 Tensors are torch tensors on the GPU; the arithmetic runs in hand-written HIP
 kernels reached through the C ABI of ``libtensorly_amd.so``
@@ -20,7 +21,7 @@ from .cp_tensor import CPTensor, cp_norm, cp_normalize, cp_to_tensor, validate_c
 from .tenalg import (khatri_rao, mode_dot, multi_mode_dot, precision, set_precision,  # noqa: F401
                      unfolding_dot_khatri_rao)
 from .tucker_tensor import TuckerTensor, tucker_to_tensor, validate_tucker_rank  # noqa: F401
-from .decomposition import CP, Tucker, parafac, partial_tucker, tucker  # noqa: F401
+from .decomposition import CP, Tucker, non_negative_parafac, parafac, partial_tucker, tucker  # noqa: F401
 
 
 def native_library_path():

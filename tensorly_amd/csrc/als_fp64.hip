@@ -321,6 +321,47 @@ __global__ __launch_bounds__(256) void cp_error_kernel(SolveArgs a, const double
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+
+// ---------------------------------------------------------------------------
+// Multiplicative update of a non-negative CP factor
+// (tensorly/decomposition/_nn_cp.py:111-135):
+//   accum = (Hadamard of the other modes' Grams) * w w^T
+//   A <- A * max(M, eps) / max(A accum, eps)
+// accum is formed in FP64 in LDS; one wave per row, lane r owns column r and
+// reads the whole row before the wave writes it back (in place).
+// ---------------------------------------------------------------------------
+struct MuArgs {
+  const double* grams[kMaxOrder];
+  int ndim, mode, R;
+  const void* weights;
+  int wdt;
+  const void* M;
+  void* A;
+  int64_t rows;
+  double eps;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void nn_mu_update_kernel(MuArgs a) {
+  __shared__ double acc[kMaxRank * kMaxRank];
+  const int R = a.R;
+  build_v(a, a.ndim, a.mode, -1, nullptr, a.weights, a.wdt, 0.0, R, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T* A = (T*)a.A;
+  const T* M = (const T*)a.M;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; i < a.rows; i += nw) {
+    const T* row = A + i * R;
+    double den = 0.0;
+    for (int k = 0; k < R; ++k) den = fma((double)row[k], lane < R ? acc[k * R + lane] : 0.0, den);
+    if (lane < R) {
+      const double num = fmax((double)M[i * R + lane], a.eps);
+      const double aval = (double)row[lane];
+      A[i * R + lane] = (T)(aval * num / fmax(den, a.eps));
+    }
+  }
+}
+
 }  // namespace tla
 
 using namespace tla;
@@ -451,3 +492,29 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
                               -1, nullptr, nullptr, stream);
 }
 
+extern "C" int tla_nn_mu_update(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
+                                const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
+                                double eps, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "nn_mu_update: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "nn_mu_update: order %d", ndim);
+  TLA_REQUIRE(mode >= 0 && mode < ndim, "nn_mu_update: mode %d", mode);
+  TLA_REQUIRE(rank >= 1 && rank <= kMaxRank, "nn_mu_update: rank %d not in [1, %d]", rank, kMaxRank);
+  TLA_REQUIRE(rows >= 0, "nn_mu_update: rows = %lld", (long long)rows);
+  TLA_REQUIRE(factor && mttkrp, "nn_mu_update: NULL pointer");
+  TLA_REQUIRE(!weights || wdtype == TLA_F32 || wdtype == TLA_F64, "nn_mu_update: bad weight dtype %d", wdtype);
+  if (rows == 0) return TLA_OK;
+  MuArgs a{};
+  for (int m = 0; m < ndim; ++m) {
+    TLA_REQUIRE(m == mode || grams[m] != nullptr, "nn_mu_update: gram %d is NULL", m);
+    a.grams[m] = grams[m];
+  }
+  a.ndim = ndim; a.mode = mode; a.R = rank;
+  a.weights = weights; a.wdt = wdtype;
+  a.M = mttkrp; a.A = factor; a.rows = rows; a.eps = eps;
+  const int64_t blocks = (rows + 3) / 4 < 1024 ? (rows + 3) / 4 : 1024;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TLA_F32) nn_mu_update_kernel<float><<<(unsigned)blocks, 256, 0, st>>>(a);
+  else nn_mu_update_kernel<double><<<(unsigned)blocks, 256, 0, st>>>(a);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}

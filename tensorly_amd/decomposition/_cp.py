@@ -58,7 +58,7 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", non_negative=Fa
         from ..base import unfold
         factors = []
         for mode in range(tensor.dim()):
-            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd)
+            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd, non_negative=non_negative)
             if mode == 0:
                 k = min(rank, S.shape[0])
                 U = U.clone()

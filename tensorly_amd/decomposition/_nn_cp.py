@@ -1,0 +1,116 @@
+"""Non-negative CP by multiplicative updates (``tensorly/decomposition/_nn_cp.py``,
+0.9.0) on the gfx950 kernels.
+
+:func:`non_negative_parafac` keeps the reference signature (``_nn_cp.py:26-40``)
+and return convention (``:174-180``).  Per mode (``:111-135``) it runs the
+fused MTTKRP (``tla_mttkrp``) and one ``tla_nn_mu_update`` launch, which forms
+``accum`` = (Hadamard of the other modes' FP64 Grams) * w w^T and applies
+``A <- A * max(M, eps) / max(A accum, eps)`` in place; the updated factor's
+FP64 Gram comes from ``tla_gram64``.  The error (``:138-150``) uses the last
+mode's MTTKRP as in CP-ALS.  ``init='svd'`` is the NNDSVDa initialisation of
+the reference (``tenalg/svd.py:68-127``) on the GPU truncated SVD.
+
+Not implemented (raise NotImplementedError rather than run elsewhere):
+``mask``.
+"""
+import warnings
+
+import torch
+
+from .. import _lib
+from .._lib import check, lib, ptr, ptr_array, stream
+from ..cp_tensor import CPTensor, cp_normalize, grams64, validate_cp_rank
+from ..tenalg import unfolding_dot_khatri_rao
+from ._cp import error_calc, initialize_cp
+from ._tucker import _sqnorm64
+
+__all__ = ["non_negative_parafac"]
+
+
+def _mu_update(factors, mttkrp, grams, weights, mode, eps):
+    f = factors[mode]
+    rows, rank = f.shape
+    m = mttkrp.contiguous()
+    gp, _keep = ptr_array(grams)
+    check(lib().tla_nn_mu_update(_lib.dtype_code(f), ptr(f), ptr(m), rows, rank, gp, len(factors), mode,
+                                 ptr(weights), _lib.dtype_code(weights), float(eps), stream(f.device)),
+          "nn_mu_update")
+
+
+def non_negative_parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", tol=10e-7,
+                         random_state=None, verbose=0, normalize_factors=False, return_errors=False, mask=None,
+                         cvg_criterion="abs_rec_error", fixed_modes=None):
+    """Non-negative CP decomposition by multiplicative updates
+    (``_nn_cp.py:26-180``); ``tensor`` is a float32 / float64 GPU tensor."""
+    _lib.require_gpu_tensor(tensor, "tensor")
+    if mask is not None:
+        raise NotImplementedError("mask is not supported by tensorly_amd.non_negative_parafac")
+    if tensor.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"non_negative_parafac takes float32 / float64 tensors, got {tensor.dtype}")
+    tensor = tensor.contiguous()
+    epsilon = torch.finfo(tensor.dtype).eps
+    rank = validate_cp_rank(tuple(tensor.shape), rank=rank)
+    weights, factors = initialize_cp(tensor, rank, init=init, svd=svd, non_negative=True, random_state=random_state,
+                                     normalize_factors=normalize_factors)
+    if weights is None:
+        weights = torch.ones(rank, dtype=tensor.dtype, device=tensor.device)
+    weights = weights.to(tensor.dtype).contiguous()
+    factors = [f.to(tensor.dtype).contiguous().clone() for f in factors]
+    rec_errors = []
+    norm_tensor = torch.sqrt(_sqnorm64(tensor))
+
+    fixed_modes = [] if fixed_modes is None else list(fixed_modes)
+    if tensor.dim() - 1 in fixed_modes:
+        warnings.warn("You asked for fixing the last mode, which is not supported while tol is fixed.\n "
+                      "The last mode will not be fixed. Consider using tl.moveaxis()")
+        fixed_modes.remove(tensor.dim() - 1)
+    modes_list = [mode for mode in range(tensor.dim()) if mode not in fixed_modes]
+
+    grams = grams64(factors)
+    mttkrp = None
+    for iteration in range(n_iter_max):
+        if verbose > 1:
+            print("Starting iteration", iteration + 1)
+        for mode in modes_list:
+            if verbose > 1:
+                print("Mode", mode, "of", tensor.dim())
+            mttkrp = unfolding_dot_khatri_rao(tensor, (weights, factors), mode)
+            _mu_update(factors, mttkrp, grams, weights, mode, epsilon)
+            if normalize_factors and mode != modes_list[-1]:
+                weights, factors = cp_normalize((weights, factors))
+                weights = weights.contiguous()
+                factors = [f.contiguous() for f in factors]
+                grams = grams64(factors)
+            else:
+                grams[mode] = grams64([factors[mode]])[0]
+
+        if tol:
+            unnorml_rec_error, _, _ = error_calc(tensor, norm_tensor, weights, factors, mttkrp=mttkrp)
+            rec_error = float(unnorml_rec_error / norm_tensor)
+            rec_errors.append(rec_error)
+            if iteration >= 1:
+                rec_error_decrease = rec_errors[-2] - rec_errors[-1]
+                if verbose:
+                    print(f"iteration {iteration}, reconstraction error: {rec_error}, "
+                          f"decrease = {rec_error_decrease}")
+                if cvg_criterion == "abs_rec_error":
+                    stop_flag = abs(rec_error_decrease) < tol
+                elif cvg_criterion == "rec_error":
+                    stop_flag = rec_error_decrease < tol
+                else:
+                    raise TypeError("Unknown convergence criterion")
+                if stop_flag:
+                    if verbose:
+                        print(f"PARAFAC converged after {iteration} iterations")
+                    break
+            elif verbose:
+                print(f"reconstruction error={rec_errors[-1]}")
+        if normalize_factors:
+            weights, factors = cp_normalize((weights, factors))
+            weights = weights.contiguous()
+            factors = [f.contiguous() for f in factors]
+            grams = grams64(factors)
+    cp_tensor = CPTensor((weights, factors))
+    if return_errors:
+        return cp_tensor, rec_errors
+    return cp_tensor

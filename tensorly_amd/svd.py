@@ -15,7 +15,8 @@ import torch
 from . import _lib
 from ._lib import check, lib, ptr, shape_array, stream, workspace
 
-__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "gram_eigh", "mode_gram", "SVD_FUNS"]
+__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd_non_negative", "gram_eigh",
+           "mode_gram", "SVD_FUNS"]
 
 SVD_FUNS = ["truncated_svd", "symeig_svd"]
 
@@ -132,13 +133,48 @@ def truncated_svd(matrix, n_eigenvecs=None, warm=None, **kwargs):
     return U, S.to(A.dtype), V
 
 
+def make_svd_non_negative(tensor, U, S, V, nntype=True):
+    """NNDSVD / NNDSVDa of ``svd.py:68-127`` on the GPU tensors of a
+    truncated SVD (column loop over the rank; init-time only)."""
+    if nntype is True:
+        nntype = "nndsvda"
+    if nntype not in ("nndsvd", "nndsvda"):
+        raise ValueError(f'Invalid nntype parameter: got {nntype} instead of one of ("nndsvd", "nndsvda")')
+    W = torch.zeros_like(U)
+    H = torch.zeros_like(V)
+    W[:, 0] = torch.sqrt(S[0]) * torch.abs(U[:, 0])
+    H[0, :] = torch.sqrt(S[0]) * torch.abs(V[0, :])
+    for j in range(1, min(U.shape[1], V.shape[0])):
+        x, y = U[:, j], V[j, :]
+        x_p, y_p = torch.clamp(x, min=0.0), torch.clamp(y, min=0.0)
+        x_n, y_n = torch.abs(torch.clamp(x, max=0.0)), torch.abs(torch.clamp(y, max=0.0))
+        x_p_nrm, y_p_nrm = torch.linalg.vector_norm(x_p), torch.linalg.vector_norm(y_p)
+        x_n_nrm, y_n_nrm = torch.linalg.vector_norm(x_n), torch.linalg.vector_norm(y_n)
+        m_p, m_n = x_p_nrm * y_p_nrm, x_n_nrm * y_n_nrm
+        if bool(m_p > m_n):
+            u, v, sigma = x_p / x_p_nrm, y_p / y_p_nrm, m_p
+        else:
+            u, v, sigma = x_n / x_n_nrm, y_n / y_n_nrm, m_n
+        lbd = torch.sqrt(S[j] * sigma)
+        W[:, j] = lbd * u
+        H[j, :] = lbd * v
+    eps = torch.finfo(tensor.dtype).eps
+    if nntype == "nndsvd":
+        W = torch.sign(W) * torch.clamp(torch.abs(W) - eps, min=0.0)
+        H = torch.sign(H) * torch.clamp(torch.abs(H) - eps, min=0.0)
+    else:
+        avg = torch.mean(tensor)
+        W = torch.where(W < eps, avg.to(W.dtype), W)
+        H = torch.where(H < eps, avg.to(H.dtype), H)
+    return W, H
+
+
 def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True, u_based_flip_sign=True,
                   non_negative=None, mask=None, n_iter_mask_imputation=5, **kwargs):
-    """``svd.py:366-447`` (no mask imputation / non-negativity)."""
+    """``svd.py:366-447`` (no mask imputation); ``non_negative`` applies
+    :func:`make_svd_non_negative` after the sign flip, as the reference."""
     if mask is not None:
         raise NotImplementedError("mask imputation is not supported by tensorly_amd")
-    if non_negative:
-        raise NotImplementedError("non-negative SVD is not supported by tensorly_amd")
     if method not in ("truncated_svd", "symeig_svd") and not callable(method):
         raise ValueError(f"Got svd={method}. However, the possible choices are {SVD_FUNS} or to pass a callable.")
     if callable(method):
@@ -147,6 +183,8 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
         U, S, V = truncated_svd(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
     if flip_sign:
         U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
+    if non_negative is not False and non_negative is not None:
+        U, V = make_svd_non_negative(matrix, U, S, V, non_negative)
     return U, S, V
 
 
