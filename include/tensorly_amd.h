@@ -177,6 +177,18 @@ int tla_nn_mu_update(int dtype, void* factor, const void* mttkrp, int64_t rows, 
                      const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
                      double eps, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Gradient of 0.5 ||X - [[w; A]]||^2 with respect to factor `mode`
+ * (tensorly/cp_tensor.py:377-430, cp_lstsq_grad without a mask), from the
+ * tensor's MTTKRP and the factors' Grams instead of the residual tensor:
+ *   grad = factor accum - mttkrp,  accum = (prod_{m != mode} G_m) * w w^T
+ * (-MTTKRP(X - M) = MTTKRP(M) - MTTKRP(X), MTTKRP(M) = factor accum).
+ * Arguments as tla_nn_mu_update; grad (rows x rank, dtype) must not alias.
+ * ------------------------------------------------------------------- */
+int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void* mttkrp, void* grad, int64_t rows,
+                           int rank, const double* const* grams, int ndim, int mode, const void* weights,
+                           int wdtype, void* stream);
+
 
 /* ---------------------------------------------------------------------
  * Leading left singular vectors through the FP64 Gram (replaces the LAPACK

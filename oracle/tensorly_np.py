@@ -33,6 +33,8 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
   exact factorisations (with non-unit weights) as fixed points of the update,
   and a perturbed factor moving towards the solution
   (``tests/test_oracle_nn_cp.py``).
+* ``cp_lstsq_grad``: central finite differences of the loss for every factor
+  entry of a small model, and the loss against ``0.5 ||X - cp_to_tensor||^2``.
 * ``make_svd_non_negative``: a non-negative rank-1 matrix reproduced exactly,
   non-negative outputs, the leading triplet kept, the NNDSVDa fill value, and
   the dominant-sign choice on a hand-built column pair.
@@ -47,7 +49,7 @@ __all__ = [
     "unfolding_dot_khatri_rao", "unfolding_dot_khatri_rao_memory",
     "unfolding_dot_khatri_rao_einsum", "mode_dot", "multi_mode_dot",
     "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
-    "cp_norm", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
+    "cp_norm", "cp_lstsq_grad", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
     "svd_interface", "make_svd_non_negative", "initialize_cp", "error_calc", "parafac",
     "non_negative_parafac",
     "validate_tucker_rank", "tucker_to_tensor", "initialize_tucker",
@@ -324,6 +326,18 @@ def cp_norm(cp_tensor):
     if weights is not None:
         acc = acc * (np.reshape(weights, (-1, 1)) * np.reshape(weights, (1, -1)))
     return np.sqrt(np.sum(acc))
+
+
+def cp_lstsq_grad(cp_tensor, tensor, return_loss=False):
+    """``cp_tensor.py:377-430`` (no mask): the residual ``X - [[w; A]]``, then
+    ``grad_n = -unfolding_dot_khatri_rao(residual, (w, A), n)`` per mode and
+    the loss ``0.5 * sum(residual ** 2)``."""
+    weights, factors = cp_tensor
+    diff = tensor - cp_to_tensor(cp_tensor)
+    grads = [-unfolding_dot_khatri_rao(diff, cp_tensor, n) for n in range(len(factors))]
+    if return_loss:
+        return (None, grads), 0.5 * np.sum(diff ** 2)
+    return None, grads
 
 
 def random_cp(shape, rank, full=False, orthogonal=False, random_state=None,

@@ -17,7 +17,7 @@ __version__ = "0.1.0"
 
 from . import _lib  # noqa: F401  (imports torch before the HIP library)
 from .base import fold, unfold  # noqa: F401
-from .cp_tensor import CPTensor, cp_norm, cp_normalize, cp_to_tensor, validate_cp_rank  # noqa: F401
+from .cp_tensor import CPTensor, cp_lstsq_grad, cp_norm, cp_normalize, cp_to_tensor, validate_cp_rank  # noqa: F401
 from .tenalg import (khatri_rao, mode_dot, multi_mode_dot, precision, set_precision,  # noqa: F401
                      unfolding_dot_khatri_rao)
 from .tucker_tensor import TuckerTensor, tucker_to_tensor, validate_tucker_rank  # noqa: F401

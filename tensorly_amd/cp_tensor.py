@@ -15,7 +15,7 @@ from ._lib import check, lib, ptr, ptr_array, stream, workspace
 from .tenalg import khatri_rao
 
 __all__ = ["CPTensor", "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
-           "cp_norm", "grams64"]
+           "cp_norm", "grams64", "cp_lstsq_grad"]
 
 
 class CPTensor(Mapping):
@@ -180,3 +180,62 @@ def cp_to_tensor(cp_tensor, mask=None):
     else:
         full = torch.sum(khatri_rao([factors[0] * weights] + list(factors[1:]), mask=mask), dim=1)
     return full.reshape(shape)
+
+
+def cp_lstsq_grad(cp_tensor, tensor, return_loss=False, mask=None):
+    """Gradient of ``0.5 ||X - [[w; A]]||^2`` with respect to each factor
+    (``cp_tensor.py:377-430``; not with respect to the weights), returned as
+    ``CPTensor((None, grads))``, and the loss when ``return_loss``.
+
+    The reference forms the residual tensor and runs one MTTKRP of it per
+    mode.  Here ``-MTTKRP(X - M, n) = A_n accum_n - MTTKRP(X, n)`` with
+    ``accum_n`` = (Hadamard of the other modes' Grams) * w w^T, so each mode
+    is one fused MTTKRP pass over X and one ``tla_cp_lstsq_grad_mode``
+    launch, and the loss is ``0.5 (||X||^2 + ||M||^2 - 2 <X, M>)`` in FP64
+    from the Grams and the last mode's MTTKRP.  No reconstruction is formed.
+    Near an exact fit both forms cancel; this one loses about
+    ``||X|| / ||X - M||`` relative digits more.  ``mask`` raises
+    NotImplementedError."""
+    from .tenalg import unfolding_dot_khatri_rao
+    if mask is not None:
+        raise NotImplementedError("mask is not supported by tensorly_amd.cp_lstsq_grad")
+    _lib.require_gpu_tensor(tensor, "tensor")
+    _, rank = validate_cp_tensor(cp_tensor)
+    weights, factors = cp_tensor
+    tensor = tensor.contiguous()
+    dt = tensor.dtype
+    if dt not in (torch.float32, torch.float64):
+        raise TypeError(f"cp_lstsq_grad takes float32 / float64 tensors, got {dt}")
+    factors = [f.to(dt).contiguous() for f in factors]
+    w = (torch.ones(rank, dtype=dt, device=tensor.device) if weights is None
+         else weights.to(dt).contiguous())
+    grams = grams64(factors)
+    gp, _keep = ptr_array(grams)
+    grads = []
+    m = None
+    for n, f in enumerate(factors):
+        m = unfolding_dot_khatri_rao(tensor, (w, factors), n).contiguous()
+        g = torch.empty_like(f)
+        check(lib().tla_cp_lstsq_grad_mode(_lib.dtype_code(f), ptr(f), ptr(m), ptr(g), f.shape[0], rank, gp,
+                                           len(factors), n, ptr(w), _lib.dtype_code(w), stream(f.device)),
+              "cp_lstsq_grad")
+        grads.append(g)
+    grad = CPTensor((None, grads))
+    if not return_loss:
+        return grad
+    nx2 = _sqnorm(tensor)
+    acc = grams[0]
+    for gm in grams[1:]:
+        acc = acc * gm
+    w64 = w.to(torch.float64)
+    nm2 = torch.sum(acc * (w64[:, None] * w64[None, :]))
+    ip = torch.sum(m.to(torch.float64) * factors[-1].to(torch.float64))
+    return grad, 0.5 * (nx2 + nm2 - 2.0 * ip)
+
+
+def _sqnorm(x):
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    nb = lib().tla_sqnorm_workspace_size(x.numel())
+    ws = workspace(nb, x.device)
+    check(lib().tla_sqnorm(_lib.dtype_code(x), ptr(x), x.numel(), ptr(out), ptr(ws), nb, stream(x.device)), "sqnorm")
+    return out[0]

@@ -339,9 +339,14 @@ struct MuArgs {
   void* A;
   int64_t rows;
   double eps;
+  void* out;  // op 1
 };
 
-template <typename T>
+// op 0: the multiplicative update above (in place).
+// op 1: out = A accum - M, the gradient of 0.5 ||X - [[w; A]]||^2 with
+//       respect to the mode's factor without forming the reconstruction
+//       (MTTKRP of the model = A accum).
+template <typename T, int OP>
 __global__ __launch_bounds__(256) void nn_mu_update_kernel(MuArgs a) {
   __shared__ double acc[kMaxRank * kMaxRank];
   const int R = a.R;
@@ -355,9 +360,13 @@ __global__ __launch_bounds__(256) void nn_mu_update_kernel(MuArgs a) {
     double den = 0.0;
     for (int k = 0; k < R; ++k) den = fma((double)row[k], lane < R ? acc[k * R + lane] : 0.0, den);
     if (lane < R) {
-      const double num = fmax((double)M[i * R + lane], a.eps);
-      const double aval = (double)row[lane];
-      A[i * R + lane] = (T)(aval * num / fmax(den, a.eps));
+      if constexpr (OP == 0) {
+        const double num = fmax((double)M[i * R + lane], a.eps);
+        const double aval = (double)row[lane];
+        A[i * R + lane] = (T)(aval * num / fmax(den, a.eps));
+      } else {
+        ((T*)a.out)[i * R + lane] = (T)(den - (double)M[i * R + lane]);
+      }
     }
   }
 }
@@ -513,8 +522,36 @@ extern "C" int tla_nn_mu_update(int dtype, void* factor, const void* mttkrp, int
   a.M = mttkrp; a.A = factor; a.rows = rows; a.eps = eps;
   const int64_t blocks = (rows + 3) / 4 < 1024 ? (rows + 3) / 4 : 1024;
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == TLA_F32) nn_mu_update_kernel<float><<<(unsigned)blocks, 256, 0, st>>>(a);
-  else nn_mu_update_kernel<double><<<(unsigned)blocks, 256, 0, st>>>(a);
+  if (dtype == TLA_F32) nn_mu_update_kernel<float, 0><<<(unsigned)blocks, 256, 0, st>>>(a);
+  else nn_mu_update_kernel<double, 0><<<(unsigned)blocks, 256, 0, st>>>(a);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+extern "C" int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void* mttkrp, void* grad, int64_t rows,
+                                      int rank, const double* const* grams, int ndim, int mode, const void* weights,
+                                      int wdtype, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_lstsq_grad: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_lstsq_grad: order %d", ndim);
+  TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_lstsq_grad: mode %d", mode);
+  TLA_REQUIRE(rank >= 1 && rank <= kMaxRank, "cp_lstsq_grad: rank %d not in [1, %d]", rank, kMaxRank);
+  TLA_REQUIRE(rows >= 0, "cp_lstsq_grad: rows = %lld", (long long)rows);
+  TLA_REQUIRE(factor && mttkrp && grad, "cp_lstsq_grad: NULL pointer");
+  TLA_REQUIRE(grad != factor && grad != mttkrp, "cp_lstsq_grad: grad must not alias its inputs");
+  TLA_REQUIRE(!weights || wdtype == TLA_F32 || wdtype == TLA_F64, "cp_lstsq_grad: bad weight dtype %d", wdtype);
+  if (rows == 0) return TLA_OK;
+  MuArgs a{};
+  for (int m = 0; m < ndim; ++m) {
+    TLA_REQUIRE(m == mode || grams[m] != nullptr, "cp_lstsq_grad: gram %d is NULL", m);
+    a.grams[m] = grams[m];
+  }
+  a.ndim = ndim; a.mode = mode; a.R = rank;
+  a.weights = weights; a.wdt = wdtype;
+  a.M = mttkrp; a.A = const_cast<void*>(factor); a.rows = rows; a.eps = 0.0; a.out = grad;
+  const int64_t blocks = (rows + 3) / 4 < 1024 ? (rows + 3) / 4 : 1024;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TLA_F32) nn_mu_update_kernel<float, 1><<<(unsigned)blocks, 256, 0, st>>>(a);
+  else nn_mu_update_kernel<double, 1><<<(unsigned)blocks, 256, 0, st>>>(a);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -127,3 +127,27 @@ def test_random_cp_draw_order():
     for s, x in zip((4, 5, 6), f):
         np.testing.assert_array_equal(x, rng.random_sample((s, 3)))
     np.testing.assert_array_equal(w, np.ones(3))
+
+
+def test_cp_lstsq_grad_matches_finite_differences():
+    """The gradient of 0.5 ||X - [[w; A]]||^2 against central differences of
+    the loss for every factor entry (weights not differentiated)."""
+    rng = np.random.default_rng(11)
+    shape, R = (3, 4, 5), 2
+    X = rng.standard_normal(shape)
+    w = rng.random(R) + 0.5
+    A = [rng.standard_normal((s, R)) for s in shape]
+    (_, grads), loss = T.cp_lstsq_grad((w, A), X, return_loss=True)
+    np.testing.assert_allclose(loss, 0.5 * np.sum((X - T.cp_to_tensor((w, A))) ** 2), rtol=1e-13)
+    h = 1e-6
+    for n in range(3):
+        fd = np.zeros_like(A[n])
+        for i in range(A[n].shape[0]):
+            for r in range(R):
+                Ap = [a.copy() for a in A]
+                Am = [a.copy() for a in A]
+                Ap[n][i, r] += h
+                Am[n][i, r] -= h
+                fd[i, r] = (0.5 * np.sum((X - T.cp_to_tensor((w, Ap))) ** 2)
+                            - 0.5 * np.sum((X - T.cp_to_tensor((w, Am))) ** 2)) / (2 * h)
+        np.testing.assert_allclose(grads[n], fd, rtol=1e-6, atol=1e-7)
