@@ -138,3 +138,16 @@ def test_planted_dropped_weight_is_caught(known_answers):
     A = np.arange(*ka["A_arange"][:2], dtype=float).reshape(ka["A_arange"][2:])
     B = np.arange(*ka["B_arange"][:2], dtype=float).reshape(ka["B_arange"][2:])
     assert not np.array_equal(T.cp_to_tensor((None, [A, B])), np.array(ka["expected"]))
+
+
+def test_kinetic_fixture_decodes():
+    """tests/golden/kinetic_slice.npz: the 32 x 12 x 10 x 30 slice of the
+    reference's kinetic data, stored exactly as 3 x in int16; missing entries
+    hold the zeros the reference's loader leaves."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kinetic_slice.npz"))
+    x = d["x3"].astype(np.float64) / 3.0
+    assert x.shape == (32, 12, 10, 30)
+    assert d["missing"].shape == x.shape and d["missing"].any()
+    assert np.all(x[d["missing"]] == 0.0)
+    assert x.max() > 100.0
