@@ -78,6 +78,42 @@ __device__ void block_gemm_f64(const double* A, int lda, const double* B, int ld
   __syncthreads();
 }
 
+// C (M x N) = A B for a row-major A in global memory with lda == K == KB:
+// each wave stages one 16-row panel of A (16 * KB contiguous doubles, read
+// with coalesced loads) in LDS and uses it for all N / 16 column tiles.
+// Read straight from memory, an MFMA A operand is 16 rows x 32 bytes per
+// load, which measured ~2.3x slower than contiguous pieces.  B (ldb) in LDS
+// or global; the same k-ascending MFMA sequence as block_gemm_f64.  stage
+// holds 16 * (KB + 1) doubles per wave.  Ends with a barrier.
+template <int KB>
+__device__ void block_gemm_f64_panel(const double* A, const double* B, int ldb, double* C, int ldc, int M, int N,
+                                     double* stage) {
+  constexpr int LDP = KB + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  double* P = stage + wave * 16 * LDP;
+  const int tn = N / 16;
+  for (int pnl = wave; pnl < M / 16; pnl += blockDim.x >> 6) {
+    const double* Ap = A + (size_t)pnl * 16 * KB;
+#pragma unroll
+    for (int e = lane; e < 16 * KB; e += 64) P[(e / KB) * LDP + (e % KB)] = Ap[e];
+    // the wave's LDS writes reach LDS before its later reads (in-order LDS
+    // queue per wave); this keeps the compiler from reordering across
+    __builtin_amdgcn_wave_barrier();
+    for (int nb = 0; nb < tn; ++nb) {
+      doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < KB; k0 += 4) {
+        const int kk = k0 + lk;
+        acc = mfma_f64(P[lr * LDP + kk], B[kk * ldb + nb * 16 + lr], acc);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) C[(pnl * 16 + lk + 4 * e) * ldc + nb * 16 + lr] = acc[e];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Jacobi on an m x m symmetric matrix A (LDS, leading dimension LDA: odd, so
 // that the column walks of a rotation hit distinct banks), accumulating the
@@ -278,7 +314,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigArgs a) {
 // barrier).  S and the factor live in Sbuf (LDS, ld b + 1) and Tbuf (ld ldt).
 template <int B>
 __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, double* Tbuf, int ldt, bool shift,
-                           double* pbuf, double* dpiv, double* scratch) {
+                           double* pbuf, double* dpiv, double* scratch, double* stage) {
   constexpr int LDS_S = B + 1, CAP = B;
   // TPR threads per row of [S | I] (rows past B idle), CPT columns each
   constexpr int TPR = B > 64 ? 4 : 8, CPT = 2 * B / TPR;
@@ -332,7 +368,8 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
     if (live && c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds;
   }
   __syncthreads();
-  block_gemm_f64<false>(M, CAP, Tbuf, ldt, Out, CAP, np, B, B);
+  if constexpr (B == 48) block_gemm_f64_panel<B>(M, Tbuf, ldt, Out, CAP, np, B, stage);
+  else block_gemm_f64<false>(M, CAP, Tbuf, ldt, Out, CAP, np, B, B);
 }
 
 // Block subspace iteration with Rayleigh-Ritz for n > kJacLarge.  B is the
@@ -353,6 +390,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
   __shared__ double theta[B];
   __shared__ double pbuf[6 * B];
   __shared__ double dpiv[B];
+  // 16-row panels of the np x B blocks for the products with B x B factors
+  __shared__ double stage[B == 48 ? kEigWaves * 16 * (B + 1) : 1];
   constexpr int LDW = SMALL ? LDH : CAP;
   double* W = SMALL ? Wl : a.W;
   double* T = SMALL ? Tl : a.T;
@@ -372,8 +411,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     Y[i * CAP + j] = i >= n ? 0.0 : ((a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j));
   }
   __syncthreads();
-  block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch);
-  block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch);
+  block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+  block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
   { double* t = V; V = Y; Y = t; }
   const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
   double jthr = 1e-10;
@@ -402,8 +441,13 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
       H[l * LDH + j] = W[l * LDW + order[j]];
     }
     __syncthreads();
-    block_gemm_f64<false>(V, CAP, H, LDH, Y, CAP, np, B, B);  // Y = V W (Ritz vectors)
-    block_gemm_f64<false>(Z, CAP, H, LDH, U, CAP, np, B, B);  // U = Z W = G Y
+    if constexpr (B == 48) {
+      block_gemm_f64_panel<B>(V, H, LDH, Y, CAP, np, B, stage);  // Y = V W (Ritz vectors)
+      block_gemm_f64_panel<B>(Z, H, LDH, U, CAP, np, B, stage);  // U = Z W = G Y
+    } else {
+      block_gemm_f64<false>(V, CAP, H, LDH, Y, CAP, np, B, B);
+      block_gemm_f64<false>(Z, CAP, H, LDH, U, CAP, np, B, B);
+    }
     // residual norms of the leading k pairs (one wave per column)
     for (int r = tid >> 6; r < k; r += nt >> 6) {
       double r2 = 0.0;
@@ -430,8 +474,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     if (done || it + 1 == a.max_iter) break;
     // V <- orth(U), two Cholesky-QR passes (the columns of U = G Y scale
     // with the Ritz values)
-    block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch);
-    block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch);
+    block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+    block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
   }
   for (int e = tid; e < n * k; e += nt) {
     const int i = e / k, r = e % k;
