@@ -487,8 +487,9 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
 
 // ---------------------------------------------------------------------------
 // Mode Gram: G[i, j] = sum_{a,b} X[a, i, b] X[a, j, b], FP64 accumulation.
-// Block (ti, tj) computes a 32 x 32 tile for one chunk of the (a, b) pairs;
-// partial tiles are summed in chunk order.
+// Block (ti, tj) computes a 32 x 32 tile for one chunk of the (a, b) pairs
+// (256 threads: four waves, one 16 x 16 FP64 MFMA tile each); partial tiles
+// are summed in chunk order.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void mode_gram_kernel(const T* __restrict__ X, int64_t Pa, int64_t In, int64_t Pb,
@@ -501,9 +502,9 @@ __global__ __launch_bounds__(256) void mode_gram_kernel(const T* __restrict__ X,
   const int64_t q0 = (int64_t)blockIdx.z * chunk;
   const int64_t Q = Pa * Pb;
   const int64_t q1 = min(Q, q0 + chunk);
-  // each thread owns 4 outputs (ri, cj..cj+3)
-  const int ri = tid / 8, cj = (tid % 8) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int mi = (wave >> 1) * 16, nj = (wave & 1) * 16;
+  doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
   for (int64_t qb = q0; qb < q1; qb += 64) {
     for (int e = tid; e < 32 * 64; e += 256) {
       const int r = e / 64, qq = e % 64;
@@ -519,22 +520,21 @@ __global__ __launch_bounds__(256) void mode_gram_kernel(const T* __restrict__ X,
       Tj[r][qq] = vj;
     }
     __syncthreads();
-    for (int qq = 0; qq < 64; ++qq) {
-      const double x = Ti[ri][qq];
+    // wave w: the 16 x 16 output tile (w / 2, w % 2) of the 32 x 32 block on
+    // FP64 MFMA (products of the float inputs are exact in FP64)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] = fma(x, Tj[cj + c][qq], acc[c]);
-    }
+    for (int k0 = 0; k0 < 64; k0 += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti[mi + lr][k0 + lk], Tj[nj + lr][k0 + lk], acc, 0, 0, 0);
     __syncthreads();
   }
-  const int64_t nt = (int64_t)gridDim.x;
   double* out = partial + (int64_t)blockIdx.z * In * In;
-  (void)nt;
+  // lane holds D[lk + 4 e][lr]
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int64_t i = (int64_t)ti * 32 + ri, j = (int64_t)tj * 32 + cj + c;
+  for (int e = 0; e < 4; ++e) {
+    const int64_t i = (int64_t)ti * 32 + mi + lk + 4 * e, j = (int64_t)tj * 32 + nj + lr;
     if (i < In && j < In) {
-      out[i * In + j] = acc[c];
-      out[j * In + i] = acc[c];
+      out[i * In + j] = acc[e];
+      out[j * In + i] = acc[e];
     }
   }
 }
