@@ -189,6 +189,24 @@ int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void* mttkrp, vo
                            int rank, const double* const* grams, int ndim, int mode, const void* weights,
                            int wdtype, void* stream);
 
+/* ---------------------------------------------------------------------
+ * HALS non-negative least squares for one CP factor
+ * (tensorly/solvers/nnls.py:5-173 hals_nnls, as called by
+ * non_negative_parafac_hals, tensorly/decomposition/_nn_cp.py:300-311):
+ * with UtU = (prod_{m != mode} G_m) * w w^T and UtM = mttkrp^T, the rows k of
+ * V = factor^T are updated in order,
+ *   V[k,:] = max((UtM[k,:] - UtU[k,:] V + UtU[k,k] V[k,:] - s) / UtU[k,k], 0)
+ * (s = sparsity when has_sparsity; rows with UtU[k,k] == 0 are skipped), for
+ * at most n_iter_max iterations, stopping when an iteration's squared update
+ * is below tol times the first iteration's.  factor (rows x rank, dtype) is
+ * updated in place, in FP64 arithmetic; iters_out (device int, nullable)
+ * receives the iterations run.  Arguments otherwise as tla_nn_mu_update.
+ * ------------------------------------------------------------------- */
+int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
+                  const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
+                  double sparsity, int has_sparsity, int n_iter_max, double tol, int* iters_out,
+                  void* stream);
+
 
 /* ---------------------------------------------------------------------
  * Leading left singular vectors through the FP64 Gram (replaces the LAPACK

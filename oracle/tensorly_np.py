@@ -33,6 +33,9 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
   exact factorisations (with non-unit weights) as fixed points of the update,
   and a perturbed factor moving towards the solution
   (``tests/test_oracle_nn_cp.py``).
+* ``hals_nnls`` / ``non_negative_parafac_hals``: the NNLS KKT conditions at
+  the ``exact`` solution, the monotone decrease of the error, and exact
+  factorisations as fixed points.
 * ``cp_lstsq_grad``: central finite differences of the loss for every factor
   entry of a small model, and the loss against ``0.5 ||X - cp_to_tensor||^2``.
 * ``make_svd_non_negative``: a non-negative rank-1 matrix reproduced exactly,
@@ -51,7 +54,7 @@ __all__ = [
     "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
     "cp_norm", "cp_lstsq_grad", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
     "svd_interface", "make_svd_non_negative", "initialize_cp", "error_calc", "parafac",
-    "non_negative_parafac",
+    "non_negative_parafac", "hals_nnls", "non_negative_parafac_hals",
     "validate_tucker_rank", "tucker_to_tensor", "initialize_tucker",
     "partial_tucker", "tucker",
 ]
@@ -676,6 +679,113 @@ def non_negative_parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncate
                 weights, factors = cp_normalize((weights, factors))
         if tol:
             rec_error = error_calc(tensor, norm_tensor, weights, factors, mttkrp=mttkrp) / norm_tensor
+            rec_errors.append(rec_error)
+            if iteration >= 1:
+                dec = rec_errors[-2] - rec_errors[-1]
+                if cvg_criterion == "abs_rec_error":
+                    stop = abs(dec) < tol
+                elif cvg_criterion == "rec_error":
+                    stop = dec < tol
+                else:
+                    raise TypeError("Unknown convergence criterion")
+                if stop:
+                    break
+        if normalize_factors:
+            weights, factors = cp_normalize((weights, factors))
+    if return_errors:
+        return (weights, factors), rec_errors
+    return weights, factors
+
+
+def hals_nnls(UtM, UtU, V=None, n_iter_max=500, tol=1e-8, sparsity_coefficient=None, ridge_coefficient=None,
+              nonzero_rows=False, exact=False, epsilon=0.0):
+    """``solvers/nnls.py:5-173``: block-coordinate (HALS) descent for
+    min_{V >= 0} ||M - U V||^2 given UtM = U^T M (rank x n) and UtU = U^T U;
+    rows of V are updated in order, each with the rows before it already
+    updated, until the squared update falls below ``tol`` times the first
+    iteration's."""
+    rank = UtM.shape[0]
+    if V is None:
+        V = np.linalg.solve(UtU, UtM)
+        V = np.clip(V, 0.0, None)
+        scale = np.sum(UtM * V) / np.sum(UtU * np.dot(V, V.T))
+        V = V * scale
+    V = np.array(V)
+    if exact:
+        n_iter_max = 50000
+        tol = 1e-16
+    rec_error0 = None
+    for iteration in range(n_iter_max):
+        rec_error = 0.0
+        for k in range(rank):
+            if UtU[k, k]:
+                num = UtM[k, :] - np.dot(UtU[k, :], V) + UtU[k, k] * V[k, :]
+                den = UtU[k, k]
+                if sparsity_coefficient is not None:
+                    num = num - sparsity_coefficient
+                if ridge_coefficient is not None:
+                    den = den + 2 * ridge_coefficient
+                newV = np.clip(num / den, epsilon, None)
+                rec_error += np.linalg.norm(V[k, :] - newV) ** 2
+                V[k, :] = newV
+                if nonzero_rows and np.all(V[k, :] == 0):
+                    V[k, :] = np.finfo(V.dtype).eps * np.max(V)
+            elif nonzero_rows:
+                raise ValueError("Column " + str(k) + " of U is zero with nonzero condition")
+        if iteration == 0:
+            rec_error0 = rec_error
+        if rec_error < tol * rec_error0:
+            break
+    return V
+
+
+def non_negative_parafac_hals(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", tol=10e-8,
+                              random_state=None, sparsity_coefficients=None, fixed_modes=None, nn_modes="all",
+                              exact=False, normalize_factors=False, return_errors=False,
+                              cvg_criterion="abs_rec_error"):
+    """``decomposition/_nn_cp.py:183-360`` (no verbose output): per mode the
+    Hadamard of the other Grams times w w^T, the MTTKRP, then
+    ``hals_nnls(MTTKRP^T, pseudo_inverse, A^T, n_iter_max=100)`` for
+    non-negative modes and a linear solve for the others; the error from the
+    last mode's MTTKRP."""
+    weights, factors = initialize_cp(tensor, rank, init=init, svd=svd, non_negative=True,
+                                     random_state=random_state, normalize_factors=normalize_factors)
+    factors = [np.array(f) for f in factors]
+    norm_tensor = norm(tensor, 2)
+    n_modes = tensor.ndim
+    if sparsity_coefficients is None or isinstance(sparsity_coefficients, float):
+        sparsity_coefficients = [sparsity_coefficients] * n_modes
+    sparsity_coefficients = list(sparsity_coefficients)
+    fixed_modes = [] if fixed_modes is None else list(fixed_modes)
+    if nn_modes == "all":
+        nn_modes = set(range(n_modes))
+    elif nn_modes is None:
+        nn_modes = set()
+    for fixed_value in fixed_modes:
+        sparsity_coefficients[fixed_value] = None
+    modes = [mode for mode in range(n_modes) if mode not in fixed_modes]
+    rec_errors = []
+    mttkrp = None
+    for iteration in range(n_iter_max):
+        for mode in modes:
+            pseudo_inverse = np.ones((rank, rank), dtype=tensor.dtype)
+            for i, factor in enumerate(factors):
+                if i != mode:
+                    pseudo_inverse = pseudo_inverse * np.dot(factor.T, factor)
+            pseudo_inverse = weights.reshape(-1, 1) * pseudo_inverse * weights.reshape(1, -1)
+            mttkrp = unfolding_dot_khatri_rao(tensor, (weights, factors), mode)
+            if mode in nn_modes:
+                nn_factor = hals_nnls(mttkrp.T, pseudo_inverse, factors[mode].T, n_iter_max=100,
+                                      sparsity_coefficient=sparsity_coefficients[mode], exact=exact)
+                factors[mode] = nn_factor.T
+            else:
+                factors[mode] = np.linalg.solve(pseudo_inverse.T, mttkrp.T).T
+            if normalize_factors and mode != modes[-1]:
+                weights, factors = cp_normalize((weights, factors))
+        if tol:
+            factors_norm = cp_norm((weights, factors))
+            iprod = np.sum(np.sum(mttkrp * factors[-1], axis=0))
+            rec_error = np.sqrt(np.abs(norm_tensor ** 2 + factors_norm ** 2 - 2 * iprod)) / norm_tensor
             rec_errors.append(rec_error)
             if iteration >= 1:
                 dec = rec_errors[-2] - rec_errors[-1]

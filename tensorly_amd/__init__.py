@@ -21,7 +21,8 @@ from .cp_tensor import CPTensor, cp_lstsq_grad, cp_norm, cp_normalize, cp_to_ten
 from .tenalg import (khatri_rao, mode_dot, multi_mode_dot, precision, set_precision,  # noqa: F401
                      unfolding_dot_khatri_rao)
 from .tucker_tensor import TuckerTensor, tucker_to_tensor, validate_tucker_rank  # noqa: F401
-from .decomposition import CP, Tucker, non_negative_parafac, parafac, partial_tucker, tucker  # noqa: F401
+from .decomposition import (CP, Tucker, non_negative_parafac, non_negative_parafac_hals, parafac,  # noqa: F401
+                            partial_tucker, tucker)
 
 
 def native_library_path():

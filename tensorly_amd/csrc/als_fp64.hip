@@ -371,6 +371,110 @@ __global__ __launch_bounds__(256) void nn_mu_update_kernel(MuArgs a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// HALS non-negative least squares for one CP factor (tensorly/solvers/
+// nnls.py:5-173 as called by non_negative_parafac_hals, _nn_cp.py:300-311):
+// UtU = (Hadamard of the other modes' Grams) * w w^T, UtM = M^T, V = A^T.
+// Each iteration updates the rows k = 0..R-1 of V in order (Gauss-Seidel),
+//   V[k,:] <- max((UtM[k,:] - UtU[k,:] V + UtU[k,k] V[k,:] - s) / UtU[k,k], 0)
+// and stops when the iteration's squared update is below tol times the
+// first iteration's.  The columns of V (rows of A) are independent within an
+// iteration: one workgroup, each thread keeps its rows' values in registers
+// for the R updates; the squared update is summed in a fixed order.
+// ---------------------------------------------------------------------------
+struct HalsArgs {
+  const double* grams[kMaxOrder];
+  int ndim, mode, R;
+  const void* weights;
+  int wdt;
+  const void* M;
+  void* A;
+  int64_t rows;
+  double sparsity;
+  int has_sparsity;
+  int n_iter_max;
+  double tol;
+  int* iters_out;
+};
+
+constexpr int kHalsThreads = 512;
+
+template <typename T, int RT>
+__global__ __launch_bounds__(kHalsThreads) void hals_nnls_kernel(HalsArgs a) {
+  __shared__ double utu[RT * RT];
+  __shared__ double acc[kMaxRank * kMaxRank];
+  __shared__ double red[kHalsThreads / 64];
+  __shared__ int stop;
+  const int R = a.R, tid = threadIdx.x;
+  build_v(a, a.ndim, a.mode, -1, nullptr, a.weights, a.wdt, 0.0, R, acc);
+  for (int e = tid; e < RT * RT; e += blockDim.x) {
+    const int k = e / RT, l = e % RT;
+    utu[e] = (k < R && l < R) ? acc[k * R + l] : 0.0;
+  }
+  __syncthreads();
+  T* A = (T*)a.A;
+  const T* M = (const T*)a.M;
+  double err0 = 0.0;
+  int it = 0;
+  for (; it < a.n_iter_max; ++it) {
+    double err = 0.0;
+    for (int64_t i = tid; i < a.rows; i += blockDim.x) {
+      // UtU is loop-invariant; an opaque copy of its address per row keeps
+      // the compiler from holding all RT * RT values in registers
+      const double* U = utu;
+      asm volatile("" : "+v"(U));
+      // at RT = 64 the MTTKRP row is read per update (v and m together would
+      // need every register)
+      constexpr bool kMRegs = RT <= 48;
+      double v[RT], m[kMRegs ? RT : 1];
+#pragma unroll
+      for (int l = 0; l < RT; ++l) {
+        v[l] = l < R ? (double)A[i * R + l] : 0.0;
+        if constexpr (kMRegs) m[l] = l < R ? (double)M[i * R + l] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < RT; ++k) {
+        const double d = U[k * RT + k];
+        if (d != 0.0) {
+          double dot = 0.0;
+#pragma unroll
+          for (int l = 0; l < RT; ++l) dot = fma(U[k * RT + l], v[l], dot);
+          double mk;
+          if constexpr (kMRegs) mk = m[k];
+          else mk = (double)M[i * R + k];  // d != 0 only for k < R
+          double num = mk - dot + d * v[k];
+          if (a.has_sparsity) num -= a.sparsity;
+          const double nv = fmax(num / d, 0.0);
+          const double diff = v[k] - nv;
+          err = fma(diff, diff, err);
+          v[k] = nv;
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < RT; ++l)
+        if (l < R) A[i * R + l] = (T)v[l];
+    }
+    err = wave_sum(err);
+    if ((tid & 63) == 0) red[tid >> 6] = err;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+      if (it == 0) red[0] = s, err0 = s;
+      stop = (s < a.tol * (it == 0 ? s : err0)) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool done = stop != 0;
+    __syncthreads();
+    if (done) {
+      ++it;
+      break;
+    }
+  }
+  if (tid == 0 && a.iters_out) *a.iters_out = it;
+}
+
 }  // namespace tla
 
 using namespace tla;
@@ -552,6 +656,48 @@ extern "C" int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void*
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TLA_F32) nn_mu_update_kernel<float, 1><<<(unsigned)blocks, 256, 0, st>>>(a);
   else nn_mu_update_kernel<double, 1><<<(unsigned)blocks, 256, 0, st>>>(a);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+extern "C" int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
+                             const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
+                             double sparsity, int has_sparsity, int n_iter_max, double tol, int* iters_out,
+                             void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "hals_nnls: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "hals_nnls: order %d", ndim);
+  TLA_REQUIRE(mode >= 0 && mode < ndim, "hals_nnls: mode %d", mode);
+  TLA_REQUIRE(rank >= 1 && rank <= kMaxRank, "hals_nnls: rank %d not in [1, %d]", rank, kMaxRank);
+  TLA_REQUIRE(rows >= 0, "hals_nnls: rows = %lld", (long long)rows);
+  TLA_REQUIRE(n_iter_max >= 0, "hals_nnls: n_iter_max = %d", n_iter_max);
+  TLA_REQUIRE(factor && mttkrp, "hals_nnls: NULL pointer");
+  TLA_REQUIRE(!weights || wdtype == TLA_F32 || wdtype == TLA_F64, "hals_nnls: bad weight dtype %d", wdtype);
+  if (rows == 0 || n_iter_max == 0) return TLA_OK;
+  HalsArgs a{};
+  for (int m = 0; m < ndim; ++m) {
+    TLA_REQUIRE(m == mode || grams[m] != nullptr, "hals_nnls: gram %d is NULL", m);
+    a.grams[m] = grams[m];
+  }
+  a.ndim = ndim; a.mode = mode; a.R = rank;
+  a.weights = weights; a.wdt = wdtype;
+  a.M = mttkrp; a.A = factor; a.rows = rows;
+  a.sparsity = sparsity; a.has_sparsity = has_sparsity;
+  a.n_iter_max = n_iter_max; a.tol = tol; a.iters_out = iters_out;
+  hipStream_t st = (hipStream_t)stream;
+  const int rt = rank <= 16 ? 16 : rank <= 32 ? 32 : rank <= 48 ? 48 : 64;
+#define TLA_HALS(T_)                                                                  \
+  switch (rt) {                                                                       \
+    case 16: hals_nnls_kernel<T_, 16><<<1, kHalsThreads, 0, st>>>(a); break;          \
+    case 32: hals_nnls_kernel<T_, 32><<<1, kHalsThreads, 0, st>>>(a); break;          \
+    case 48: hals_nnls_kernel<T_, 48><<<1, kHalsThreads, 0, st>>>(a); break;          \
+    default: hals_nnls_kernel<T_, 64><<<1, kHalsThreads, 0, st>>>(a); break;          \
+  }
+  if (dtype == TLA_F32) {
+    TLA_HALS(float)
+  } else {
+    TLA_HALS(double)
+  }
+#undef TLA_HALS
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -1,6 +1,7 @@
 """Decomposition drop-ins: :func:`parafac` / :class:`CP` (CP-ALS),
-:func:`non_negative_parafac` (multiplicative updates) and :func:`tucker` /
+:func:`non_negative_parafac` (multiplicative updates),
+:func:`non_negative_parafac_hals` (HALS) and :func:`tucker` /
 :func:`partial_tucker` / :class:`Tucker` (HOOI)."""
 from ._cp import CP, error_calc, initialize_cp, parafac  # noqa: F401
-from ._nn_cp import non_negative_parafac  # noqa: F401
+from ._nn_cp import non_negative_parafac, non_negative_parafac_hals  # noqa: F401
 from ._tucker import Tucker, initialize_tucker, partial_tucker, tucker  # noqa: F401

@@ -104,3 +104,69 @@ def test_non_negative_parafac_rejects_mask(cuda):
     X = torch.rand((4, 5, 6), device=cuda, dtype=torch.float64)
     with pytest.raises(NotImplementedError):
         tla.non_negative_parafac(X, 2, mask=torch.ones_like(X))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rows,rank", [(1, 1), (37, 5), (700, 16), (129, 33), (64, 64)])
+@pytest.mark.parametrize("sparsity", [None, 0.05])
+def test_hals_nnls_kernel_vs_oracle(cuda, dtype, rows, rank, sparsity):
+    from tensorly_amd.cp_tensor import grams64
+    from tensorly_amd.decomposition._nn_cp import _hals_update
+    rng = np.random.RandomState(rows * 7 + rank)
+    shape = (rows, 11, 9)
+    factors = [torch.tensor(rng.random_sample((s, rank)), dtype=dtype, device=cuda) for s in shape]
+    w = torch.tensor(rng.random_sample(rank) + 0.5, dtype=dtype, device=cuda)
+    mttkrp = torch.tensor(rng.standard_normal((rows, rank)) * 3.0, dtype=dtype, device=cuda)
+    A0 = factors[0].double().cpu().numpy()
+    grams = grams64(factors)
+    _hals_update(factors, mttkrp, grams, w, 0, sparsity, 100, 1e-8)
+    Fn = [f.double().cpu().numpy() for f in factors]
+    wn = w.double().cpu().numpy()
+    utu = wn[:, None] * ((Fn[1].T @ Fn[1]) * (Fn[2].T @ Fn[2])) * wn[None, :]
+    ref = T.hals_nnls(mttkrp.double().cpu().numpy().T, utu, A0.T, n_iter_max=100, tol=1e-8,
+                      sparsity_coefficient=sparsity).T
+    got = factors[0].double().cpu().numpy()
+    assert (got >= 0).all()
+    if dtype == torch.float64:
+        np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    else:
+        assert np.linalg.norm(got - ref) <= 1e-4 * max(np.linalg.norm(ref), 1e-30)
+
+
+@pytest.mark.parametrize("init", ["random", "svd"])
+@pytest.mark.parametrize("kw", [{}, {"normalize_factors": True}, {"fixed_modes": [0]}, {"nn_modes": [0, 2]},
+                                {"sparsity_coefficients": [0.01, None, 0.02]}])
+def test_non_negative_parafac_hals_float64_vs_oracle(cuda, init, kw):
+    import tensorly_amd as tla
+    X = _data(6, (14, 12, 16), 3)
+    cp, errs = tla.non_negative_parafac_hals(torch.tensor(X, device=cuda), 3, n_iter_max=15, init=init,
+                                             random_state=4, tol=1e-300, return_errors=True, **kw)
+    (w_r, f_r), errs_r = T.non_negative_parafac_hals(X, 3, n_iter_max=15, init=init, random_state=4, tol=1e-300,
+                                                     return_errors=True, **kw)
+    # tol=1e-300 stops only at an exactly repeated error, which the GPU's and
+    # the oracle's last digits can reach one iteration apart
+    n = min(len(errs), len(errs_r))
+    assert n >= 10 and abs(len(errs) - len(errs_r)) <= 1
+    np.testing.assert_allclose(errs[:n], errs_r[:n], rtol=1e-8 if init == "random" else 1e-5)
+    for f, fr, m in zip(cp.factors, f_r, range(3)):
+        a = f.cpu().numpy()
+        if "nn_modes" not in kw or m in kw["nn_modes"]:
+            assert (a >= 0).all()
+        np.testing.assert_allclose(a, fr, rtol=1e-6 if init == "random" else 1e-3,
+                                   atol=1e-8 * np.abs(fr).max() if init == "random" else 1e-4 * np.abs(fr).max())
+
+
+def test_non_negative_parafac_hals_exact_and_float32(cuda):
+    import tensorly_amd as tla
+    X = _data(7, (10, 9, 12), 2)
+    cp, errs = tla.non_negative_parafac_hals(torch.tensor(X, device=cuda), 2, n_iter_max=5, init="random",
+                                             random_state=0, tol=1e-300, exact=True, return_errors=True)
+    _, errs_r = T.non_negative_parafac_hals(X, 2, n_iter_max=5, init="random", random_state=0, tol=1e-300,
+                                            exact=True, return_errors=True)
+    np.testing.assert_allclose(errs, errs_r, rtol=1e-7)
+    cp32, errs32 = tla.non_negative_parafac_hals(torch.tensor(X, dtype=torch.float32, device=cuda), 2,
+                                                 n_iter_max=20, init="random", random_state=0, tol=1e-300,
+                                                 return_errors=True)
+    _, errs_r32 = T.non_negative_parafac_hals(X, 2, n_iter_max=20, init="random", random_state=0, tol=1e-300,
+                                              return_errors=True)
+    np.testing.assert_allclose(errs32, errs_r32, atol=2e-5)

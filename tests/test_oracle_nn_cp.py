@@ -100,3 +100,41 @@ def test_nndsvd_column_takes_the_dominant_sign_part():
     eps = np.finfo(np.float64).eps
     np.testing.assert_allclose(W[:, 1], np.clip(lbd * x_p / np.linalg.norm(x_p) - eps, 0, None), rtol=1e-12)
     np.testing.assert_allclose(H[1], np.clip(lbd * y_p / np.linalg.norm(y_p) - eps, 0, None), rtol=1e-12)
+
+
+def test_hals_nnls_exact_meets_the_kkt_conditions():
+    """min_{V >= 0} ||M - U V||^2 at the solution: V >= 0, the gradient
+    UtU V - UtM vanishes where V > 0 and is >= 0 where V = 0."""
+    rng = np.random.RandomState(12)
+    U = rng.random_sample((30, 5))
+    M = rng.standard_normal((30, 8))
+    UtU, UtM = U.T @ U, U.T @ M
+    V = T.hals_nnls(UtM, UtU, rng.random_sample((5, 8)), exact=True)
+    grad = UtU @ V - UtM
+    scale = np.abs(UtM).max()
+    assert (V >= 0).all() and (V == 0).any()
+    # exact mode stops when an iteration's squared update is 1e-16 of the
+    # first one's, which leaves gradients of ~1e-8 relative
+    assert np.abs(grad[V > 0]).max() < 1e-6 * scale
+    assert grad[V == 0].min() > -1e-6 * scale
+
+
+@pytest.mark.parametrize("init", ["random", "svd"])
+def test_hals_error_never_increases(init):
+    """Each row update of HALS minimises the objective exactly over that row
+    (within the constraint), so the error never increases."""
+    X, _ = _positive_model(13, (8, 9, 10), 3)
+    X = X + 0.05 * np.random.RandomState(14).random_sample(X.shape)
+    _, errs = T.non_negative_parafac_hals(X, 4, n_iter_max=40, init=init, random_state=2, tol=1e-300,
+                                          return_errors=True)
+    assert len(errs) == 40
+    assert np.all(np.diff(errs) <= 1e-12 * errs[0])
+
+
+def test_hals_leaves_an_exact_factorisation_unchanged():
+    X, A = _positive_model(15, (6, 7, 8), 3)
+    (w, A2), errs = T.non_negative_parafac_hals(X, 3, n_iter_max=3, init=(np.ones(3), [a.copy() for a in A]),
+                                               tol=1e-300, return_errors=True)
+    for a, b in zip(A2, A):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+    assert max(errs) < 1e-7
