@@ -23,7 +23,7 @@ Also reported on the same JSON line:
     (HIP events on the launch stream) against 8 TB/s;
   * cpu_baseline: the numpy restatement of the reference path (unfold copy,
     materialised Khatri-Rao, np.dot, Hadamard of Grams, solve) timed on the
-    host for one full sweep in float32.
+    host in float32, per full sweep and per MTTKRP of each mode.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -457,7 +457,8 @@ def batched_report(torch, dev, net, log):
 
 def cpu_baseline(x32):
     """Float32 ALS sweeps of the numpy restatement on the host cores: the
-    median over about 10 s of sweeps after a warm-up."""
+    median over about 10 s of sweeps after a warm-up, and per mode the
+    median of 3 MTTKRPs after a warm-up."""
     from threadpoolctl import threadpool_limits
     from oracle import tensorly_np as T
     cores = os.cpu_count() or 1
@@ -483,9 +484,21 @@ def cpu_baseline(x32):
             t0 = time.perf_counter()
             sweep()
             ts.append(time.perf_counter() - t0)
+        # per-MTTKRP: median of 3 after a warm-up, per mode (unfold copy +
+        # materialised Khatri-Rao + np.dot, as in the sweep)
+        per_mode = {}
+        for mode in range(3):
+            T.unfolding_dot_khatri_rao(x32, (w, factors), mode)
+            tm = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                T.unfolding_dot_khatri_rao(x32, (w, factors), mode)
+                tm.append(time.perf_counter() - t0)
+            per_mode[f"mode{mode}"] = float(np.median(tm))
     t = float(np.median(ts))
     return {"value": round(3 * MTTKRP_BYTES / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "seconds_per_sweep": t, "time_to_fit_seconds_100_sweeps": 100 * t,
+            "seconds_per_sweep": t, "seconds_per_mttkrp": per_mode,
+            "time_to_fit_seconds_100_sweeps": 100 * t,
             "sample": f"float32 CP-ALS sweeps (3 MTTKRPs with unfold copy + materialised Khatri-Rao + np.dot, "
                       f"Hadamard of Grams, np.linalg.solve, fit) of the numpy restatement on the 512^3 tensor: "
                       f"median of {len(ts)} sweeps after 1 warm-up ({sum(ts):.1f} s of CPU work), "
