@@ -94,6 +94,24 @@ int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const 
                           size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
+ * MTTKRP of a sparse tensor in coordinate form (the unfolding_dot_khatri_rao
+ * of tensorly/contrib/sparse/cp_tensor.py:9, there over the pydata `sparse`
+ * backend): out(I_mode x R)[i, r] = w[r] * sum over nonzeros with index i in
+ * `mode` of val * prod_{m != mode} factors[m][i_m, r].
+ *   coords  device int64 [ndim][nnz] (mode-major); values nnz in dtype;
+ *   perm    device int64 nnz: the nonzeros ordered by their `mode` index
+ *           (a stable order makes the result repeatable bit for bit);
+ *   row_ptr device int64 shape[mode] + 1: row i's nonzeros are
+ *           perm[row_ptr[i] .. row_ptr[i+1]);
+ *   factors host array of ndim device pointers (entry `mode` unused), dtype;
+ *   weights R values (dtype) or NULL.  rank <= 64.
+ * ------------------------------------------------------------------- */
+int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* shape, const int64_t* coords,
+                   const void* values, const int64_t* perm, const int64_t* row_ptr,
+                   const void* const* factors, const void* weights, int rank, int mode, void* out,
+                   void* stream);
+
+/* ---------------------------------------------------------------------
  * Khatri-Rao product (materialised, for the khatri_rao API only).
  * Replaces tensorly/tenalg/core_tenalg/_khatri_rao.py:9-109.
  *   matrices host array of n device pointers (rows[i] x R, row-major);

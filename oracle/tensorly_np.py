@@ -36,6 +36,8 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
 * ``hals_nnls`` / ``non_negative_parafac_hals``: the NNLS KKT conditions at
   the ``exact`` solution, the monotone decrease of the error, and exact
   factorisations as fixed points.
+* ``unfolding_dot_khatri_rao_coo``: the dense MTTKRP of the densified tensor
+  (repeated coordinates summed).
 * ``cp_lstsq_grad``: central finite differences of the loss for every factor
   entry of a small model, and the loss against ``0.5 ||X - cp_to_tensor||^2``.
 * ``make_svd_non_negative``: a non-negative rank-1 matrix reproduced exactly,
@@ -50,7 +52,7 @@ import numpy as np
 __all__ = [
     "check_random_state", "norm", "unfold", "fold", "khatri_rao",
     "unfolding_dot_khatri_rao", "unfolding_dot_khatri_rao_memory",
-    "unfolding_dot_khatri_rao_einsum", "mode_dot", "multi_mode_dot",
+    "unfolding_dot_khatri_rao_einsum", "unfolding_dot_khatri_rao_coo", "mode_dot", "multi_mode_dot",
     "validate_cp_tensor", "validate_cp_rank", "cp_normalize", "cp_to_tensor",
     "cp_norm", "cp_lstsq_grad", "random_cp", "svd_flip", "truncated_svd", "symeig_svd",
     "svd_interface", "make_svd_non_negative", "initialize_cp", "error_calc", "parafac",
@@ -186,6 +188,24 @@ def unfolding_dot_khatri_rao_einsum(tensor, cp_tensor, mode):
             subs.append(letters[i] + "r")
     spec = ",".join(subs) + "->" + letters[mode] + "r"
     return np.einsum(spec, *ops)
+
+
+def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
+    """MTTKRP of a sparse tensor in coordinate form (``contrib/sparse/
+    cp_tensor.py:9``: ``unfolding_dot_khatri_rao`` over a sparse backend):
+    each nonzero adds ``val * w * prod_{m != mode} A_m[i_m, :]`` to row
+    ``i_mode`` (repeated coordinates add up, as when densified)."""
+    weights, factors = cp_tensor
+    rank = factors[0].shape[1]
+    contrib = np.asarray(values, dtype=np.float64)[:, None] * np.ones((1, rank))
+    for m in range(len(shape)):
+        if m != mode:
+            contrib = contrib * factors[m][coords[m]]
+    if weights is not None:
+        contrib = contrib * weights[None, :]
+    out = np.zeros((shape[mode], rank))
+    np.add.at(out, coords[mode], contrib)
+    return out
 
 
 def mode_dot(tensor, matrix_or_vector, mode, transpose=False):

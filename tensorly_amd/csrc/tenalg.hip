@@ -481,6 +481,54 @@ static int launch_ttm_cols(const TtmColsArgs& a, hipStream_t st) {
   return TLA_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// MTTKRP of a sparse tensor in coordinate (COO) form: the nonzeros come
+// ordered by their mode-n index (perm, with row_ptr delimiting each output
+// row); one wave per output row, lane r accumulating
+//   val * w[r] * prod_{m != n} A_m[i_m, r]
+// over the row's nonzeros in that order (no atomics: repeatable bit for bit).
+// ---------------------------------------------------------------------------
+struct CooArgs {
+  const int64_t* coords;  // [ndim][nnz]
+  int64_t nnz;
+  const void* vals;
+  const int64_t* perm;
+  const int64_t* row_ptr;
+  const void* factors[kMaxOrder];
+  const void* weights;
+  int ndim, mode, R;
+  int64_t rows;
+  void* out;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void mttkrp_coo_kernel(CooArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const T* vals = (const T*)a.vals;
+  const T* w = (const T*)a.weights;
+  const int R = a.R;
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < a.rows; i += nw) {
+    const int64_t p0 = a.row_ptr[i], p1 = a.row_ptr[i + 1];
+    T acc = (T)0;
+    if (lane < R) {
+      for (int64_t p = p0; p < p1; ++p) {
+        const int64_t q = a.perm[p];
+        T t = vals[q];
+        for (int m = 0; m < a.ndim; ++m) {
+          if (m == a.mode) continue;
+          const int64_t im = a.coords[(int64_t)m * a.nnz + q];
+          t *= ((const T*)a.factors[m])[im * R + lane];
+        }
+        acc += t;
+      }
+      if (w) acc *= w[lane];
+      ((T*)a.out)[i * R + lane] = acc;
+    }
+  }
+}
+
 }  // namespace tla
 
 using namespace tla;
@@ -588,6 +636,33 @@ extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int r
     return TLA_OK;
   }
   rankdiag_axis0_kernel<<<(unsigned)ceil_div(n, 64), 256, 0, st>>>(Y, I, J, rank, F, out);
+  TLA_LAUNCH_CHECK();
+  return TLA_OK;
+}
+
+extern "C" int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* shape, const int64_t* coords,
+                              const void* values, const int64_t* perm, const int64_t* row_ptr,
+                              const void* const* factors, const void* weights, int rank, int mode, void* out,
+                              void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "mttkrp_coo: bad dtype %d", dtype);
+  TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "mttkrp_coo: order %d", ndim);
+  TLA_REQUIRE(mode >= 0 && mode < ndim, "mttkrp_coo: mode %d", mode);
+  TLA_REQUIRE(rank >= 1 && rank <= 64, "mttkrp_coo: rank %d not in [1, 64]", rank);
+  TLA_REQUIRE(nnz >= 0, "mttkrp_coo: nnz = %lld", (long long)nnz);
+  TLA_REQUIRE(shape && out && row_ptr, "mttkrp_coo: NULL pointer");
+  TLA_REQUIRE(nnz == 0 || (coords && values && perm), "mttkrp_coo: NULL nonzero arrays");
+  CooArgs a{};
+  for (int m = 0; m < ndim; ++m) {
+    TLA_REQUIRE(shape[m] >= 1, "mttkrp_coo: shape[%d] = %lld", m, (long long)shape[m]);
+    TLA_REQUIRE(m == mode || factors[m] != nullptr, "mttkrp_coo: factor %d is NULL", m);
+    a.factors[m] = factors[m];
+  }
+  a.coords = coords; a.nnz = nnz; a.vals = values; a.perm = perm; a.row_ptr = row_ptr;
+  a.weights = weights; a.ndim = ndim; a.mode = mode; a.R = rank; a.rows = shape[mode]; a.out = out;
+  const int64_t blocks = std::min<int64_t>((a.rows + 3) / 4, 4096);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TLA_F32) mttkrp_coo_kernel<float><<<(unsigned)blocks, 256, 0, st>>>(a);
+  else mttkrp_coo_kernel<double><<<(unsigned)blocks, 256, 0, st>>>(a);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

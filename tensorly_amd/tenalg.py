@@ -27,7 +27,7 @@ import torch
 from . import _lib
 from ._lib import check, lib, ptr, ptr_array, shape_array, stream, workspace
 
-__all__ = ["khatri_rao", "unfolding_dot_khatri_rao", "mode_dot", "multi_mode_dot",
+__all__ = ["khatri_rao", "unfolding_dot_khatri_rao", "unfolding_dot_khatri_rao_coo", "mode_dot", "multi_mode_dot",
            "set_precision", "get_precision", "precision", "register", "mttkrp_partial",
            "rankdiag_contract"]
 
@@ -146,6 +146,47 @@ def unfolding_dot_khatri_rao(tensor, cp_tensor, mode):
     pp, _keep = ptr_array(facs)
     check(lib().tla_mttkrp(xcode, ptr(x), nd, sh, pp, ptr(w), rank, mode, prec, ptr(out), ptr(ws),
                            nbytes, stream(x.device)), "unfolding_dot_khatri_rao")
+    return out
+
+
+def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
+    """MTTKRP of a sparse tensor given in coordinate form: ``coords`` (ndim x
+    nnz int64 GPU tensor), ``values`` (nnz) and ``shape``.  Same result as
+    :func:`unfolding_dot_khatri_rao` of the densified tensor (the path of
+    ``tensorly/contrib/sparse/cp_tensor.py:9`` over the pydata ``sparse``
+    backend), without densifying anything: the nonzeros are ordered by their
+    ``mode`` index (a stable sort, so the result repeats bit for bit) and
+    ``tla_mttkrp_coo`` sums each output row in that order."""
+    weights, factors = cp_tensor
+    _lib.require_gpu_tensor(values, "values")
+    if coords.dim() != 2 or coords.shape[0] != len(shape) or coords.shape[1] != values.shape[0]:
+        raise ValueError(f"coords must be (ndim, nnz) = ({len(shape)}, {values.shape[0]}), got {tuple(coords.shape)}")
+    if len(factors) != len(shape):
+        raise ValueError(f"{len(factors)} factors for an order-{len(shape)} tensor")
+    dt = values.dtype
+    if dt not in (torch.float32, torch.float64):
+        raise TypeError(f"unfolding_dot_khatri_rao_coo takes float32 / float64 values, got {dt}")
+    rank = int(factors[0].shape[1])
+    coords = coords.to(device=values.device, dtype=torch.int64).contiguous()
+    nnz = int(values.shape[0])
+    for m, s in enumerate(shape):
+        if m != mode and tuple(factors[m].shape) != (int(s), rank):
+            raise ValueError(f"factor {m} has shape {tuple(factors[m].shape)}, expected ({int(s)}, {rank})")
+    if nnz and (int(coords.min()) < 0 or any(int(coords[m].max()) >= int(s) for m, s in enumerate(shape))):
+        raise ValueError("coordinates out of the tensor's shape")
+    idx = coords[mode]
+    perm = torch.argsort(idx, stable=True) if nnz else torch.zeros(0, dtype=torch.int64, device=values.device)
+    row_ptr = torch.zeros(int(shape[mode]) + 1, dtype=torch.int64, device=values.device)
+    if nnz:
+        row_ptr[1:] = torch.cumsum(torch.bincount(idx, minlength=int(shape[mode])), 0)
+    facs = [None if m == mode else factors[m].to(dt).contiguous() for m in range(len(shape))]
+    w = None if weights is None else weights.to(dt).contiguous()
+    out = torch.empty((int(shape[mode]), rank), dtype=dt, device=values.device)
+    sh, _keep_sh = shape_array(shape)
+    pp, _keep_f = ptr_array(facs)
+    check(lib().tla_mttkrp_coo(_lib.dtype_code(values), nnz, len(shape), sh, ptr(coords), ptr(values.contiguous()),
+                               ptr(perm), ptr(row_ptr), pp, ptr(w), rank, mode, ptr(out), stream(values.device)),
+          "mttkrp_coo")
     return out
 
 

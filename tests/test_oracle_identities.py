@@ -151,3 +151,20 @@ def test_cp_lstsq_grad_matches_finite_differences():
                 fd[i, r] = (0.5 * np.sum((X - T.cp_to_tensor((w, Ap))) ** 2)
                             - 0.5 * np.sum((X - T.cp_to_tensor((w, Am))) ** 2)) / (2 * h)
         np.testing.assert_allclose(grads[n], fd, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("shape", [(7, 8, 9), (4, 5, 3, 6)])
+def test_coo_mttkrp_matches_the_densified_tensor(shape):
+    rng = np.random.default_rng(21)
+    nnz = 60
+    coords = np.stack([rng.integers(0, s, nnz) for s in shape])
+    coords[:, 1] = coords[:, 0]  # one repeated coordinate: its values add up
+    vals = rng.standard_normal(nnz)
+    dense = np.zeros(shape)
+    np.add.at(dense, tuple(coords), vals)
+    R = 3
+    w = rng.random(R) + 0.5
+    factors = [rng.standard_normal((s, R)) for s in shape]
+    for mode in range(len(shape)):
+        np.testing.assert_allclose(T.unfolding_dot_khatri_rao_coo(coords, vals, shape, (w, factors), mode),
+                                   _brute_mttkrp(dense, w, factors, mode), rtol=1e-12, atol=1e-12)
