@@ -22,7 +22,7 @@ from .tenalg import (khatri_rao, mode_dot, multi_mode_dot, precision, set_precis
                      unfolding_dot_khatri_rao)
 from .tucker_tensor import TuckerTensor, tucker_to_tensor, validate_tucker_rank  # noqa: F401
 from .decomposition import (CP, Tucker, non_negative_parafac, non_negative_parafac_hals, parafac,  # noqa: F401
-                            partial_tucker, tucker)
+                            parafac_coo, partial_tucker, tucker)
 
 
 def native_library_path():

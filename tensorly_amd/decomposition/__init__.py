@@ -4,4 +4,5 @@
 :func:`partial_tucker` / :class:`Tucker` (HOOI)."""
 from ._cp import CP, error_calc, initialize_cp, parafac  # noqa: F401
 from ._nn_cp import non_negative_parafac, non_negative_parafac_hals  # noqa: F401
+from ._sparse_cp import parafac_coo  # noqa: F401
 from ._tucker import Tucker, initialize_tucker, partial_tucker, tucker  # noqa: F401

@@ -72,3 +72,41 @@ def test_coo_mttkrp_rejects_out_of_range_coordinates(cuda):
     ft = [torch.ones((s, 2), dtype=torch.float64, device=cuda) for s in shape]
     with pytest.raises(ValueError, match="out of the tensor"):
         tla.tenalg.unfolding_dot_khatri_rao_coo(coords, v, shape, (None, ft), 0)
+
+
+@pytest.mark.parametrize("shape,nnz", [((20, 25, 30), 900), ((8, 9, 10, 7), 600)])
+def test_parafac_coo_matches_dense_oracle(cuda, shape, nnz):
+    """CP-ALS on the COO form follows the oracle's dense parafac on the
+    densified tensor (same init draws), repeated coordinates included."""
+    import tensorly_amd as tla
+    coords, vals = _coo(nnz, shape, nnz)
+    dense = np.zeros(shape)
+    np.add.at(dense, tuple(coords), vals)
+    cp, errs = tla.parafac_coo(torch.tensor(coords, device=cuda), torch.tensor(vals, device=cuda), shape, 3,
+                               n_iter_max=8, init="random", random_state=5, tol=0, return_errors=True)
+    (w_r, f_r), errs_r = T.parafac(dense, 3, n_iter_max=8, init="random", random_state=5, tol=0,
+                                   return_errors=True)
+    np.testing.assert_allclose(errs, errs_r, rtol=1e-9)
+    rec = T.cp_to_tensor((cp.weights.cpu().numpy(), [f.cpu().numpy() for f in cp.factors]))
+    ref = T.cp_to_tensor((w_r, f_r))
+    assert np.linalg.norm(rec - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_parafac_coo_stays_sparse(cuda):
+    """The reference's sparse parafac case: a 2862 x 14036 sparse matrix,
+    rank 3, init='random' (seed 1234), one iteration; nothing is densified."""
+    import tensorly_amd as tla
+    shape = (2862, 14036)
+    coords, vals = _coo(1234, shape, 400_000, dup=False)
+    vals = np.abs(vals)
+    cp, errs = tla.parafac_coo(torch.tensor(coords, device=cuda), torch.tensor(vals, device=cuda), shape, 3,
+                               n_iter_max=1, init="random", random_state=1234, tol=0, return_errors=True)
+    assert len(errs) == 1 and 0.0 < errs[0] < 1.0
+    assert all(bool(torch.isfinite(f).all()) for f in cp.factors)
+
+
+def test_parafac_coo_rejects_svd_init(cuda):
+    import tensorly_amd as tla
+    with pytest.raises(NotImplementedError):
+        tla.parafac_coo(torch.zeros((3, 1), dtype=torch.int64, device=cuda), torch.ones(1, device=cuda,
+                        dtype=torch.float64), (2, 2, 2), 1, init="svd")
