@@ -98,18 +98,25 @@ int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const 
  * of tensorly/contrib/sparse/cp_tensor.py:9, there over the pydata `sparse`
  * backend): out(I_mode x R)[i, r] = w[r] * sum over nonzeros with index i in
  * `mode` of val * prod_{m != mode} factors[m][i_m, r].
- *   coords  device int64 [ndim][nnz] (mode-major); values nnz in dtype;
- *   perm    device int64 nnz: the nonzeros ordered by their `mode` index
- *           (a stable order makes the result repeatable bit for bit);
- *   row_ptr device int64 shape[mode] + 1: row i's nonzeros are
- *           perm[row_ptr[i] .. row_ptr[i+1]);
- *   factors host array of ndim device pointers (entry `mode` unused), dtype;
- *   weights R values (dtype) or NULL.  rank <= 64.
+ *   coords   device int64 [ndim][nnz] (mode-major); values nnz in dtype;
+ *   perm     device int64 nnz: the nonzeros ordered by their `mode` index
+ *            (a stable order makes the result repeatable bit for bit);
+ *   row_ptr  device int64 shape[mode] + 1: row i's nonzeros are
+ *            perm[row_ptr[i] .. row_ptr[i+1]);
+ *   segments each row's nonzeros cut into n_seg pieces of at most seg_len:
+ *            seg_row[j] / seg_start[j] (device int64, n_seg) give piece j's
+ *            row and first position in perm; seg_row_ptr (shape[mode] + 1)
+ *            delimits each row's pieces;
+ *   factors  host array of ndim device pointers (entry `mode` unused), dtype;
+ *   weights  R values (dtype) or NULL.  rank <= 64.
+ *   workspace  tla_mttkrp_coo_workspace_size(dtype, n_seg, rank) bytes.
  * ------------------------------------------------------------------- */
+size_t tla_mttkrp_coo_workspace_size(int dtype, int64_t n_seg, int rank);
 int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* shape, const int64_t* coords,
-                   const void* values, const int64_t* perm, const int64_t* row_ptr,
-                   const void* const* factors, const void* weights, int rank, int mode, void* out,
-                   void* stream);
+                   const void* values, const int64_t* perm, const int64_t* row_ptr, int64_t n_seg,
+                   const int64_t* seg_row, const int64_t* seg_start, const int64_t* seg_row_ptr,
+                   int64_t seg_len, const void* const* factors, const void* weights, int rank, int mode,
+                   void* out, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Khatri-Rao product (materialised, for the khatri_rao API only).

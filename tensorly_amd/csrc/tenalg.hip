@@ -483,36 +483,46 @@ static int launch_ttm_cols(const TtmColsArgs& a, hipStream_t st) {
 
 
 // ---------------------------------------------------------------------------
-// MTTKRP of a sparse tensor in coordinate (COO) form: the nonzeros come
-// ordered by their mode-n index (perm, with row_ptr delimiting each output
-// row); one wave per output row, lane r accumulating
-//   val * w[r] * prod_{m != n} A_m[i_m, r]
-// over the row's nonzeros in that order (no atomics: repeatable bit for bit).
+// MTTKRP of a sparse tensor in coordinate (COO) form.  The nonzeros come
+// ordered by their mode-n index (perm; row_ptr delimits each output row) and
+// cut into segments of at most seg_len nonzeros within a row, so that a mode
+// with few rows still spreads over many waves:
+//   pass 1, one wave per segment: partial[j][r] = sum over the segment, in
+//           order, of val * prod_{m != n} A_m[i_m, r];
+//   pass 2, one wave per row: out[i][r] = w[r] * sum of the row's segment
+//           partials in segment order.
+// No atomics: repeatable bit for bit.
 // ---------------------------------------------------------------------------
 struct CooArgs {
   const int64_t* coords;  // [ndim][nnz]
   int64_t nnz;
   const void* vals;
   const int64_t* perm;
-  const int64_t* row_ptr;
+  const int64_t* row_ptr;      // rows + 1
+  const int64_t* seg_row;      // n_seg
+  const int64_t* seg_start;    // n_seg
+  const int64_t* seg_row_ptr;  // rows + 1
+  int64_t n_seg, seg_len;
   const void* factors[kMaxOrder];
   const void* weights;
   int ndim, mode, R;
   int64_t rows;
+  void* partial;  // n_seg x R
   void* out;
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void mttkrp_coo_kernel(CooArgs a) {
+__global__ __launch_bounds__(256) void mttkrp_coo_segments_kernel(CooArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
   const T* vals = (const T*)a.vals;
-  const T* w = (const T*)a.weights;
   const int R = a.R;
-  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < a.rows; i += nw) {
-    const int64_t p0 = a.row_ptr[i], p1 = a.row_ptr[i + 1];
-    T acc = (T)0;
+  for (int64_t j = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < a.n_seg; j += nw) {
+    const int64_t row = a.seg_row[j];
+    const int64_t p0 = a.seg_start[j];
+    const int64_t p1 = min(p0 + a.seg_len, a.row_ptr[row + 1]);
     if (lane < R) {
+      T acc = (T)0;
       for (int64_t p = p0; p < p1; ++p) {
         const int64_t q = a.perm[p];
         T t = vals[q];
@@ -523,6 +533,21 @@ __global__ __launch_bounds__(256) void mttkrp_coo_kernel(CooArgs a) {
         }
         acc += t;
       }
+      ((T*)a.partial)[j * R + lane] = acc;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mttkrp_coo_rows_kernel(CooArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const T* w = (const T*)a.weights;
+  const int R = a.R;
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < a.rows; i += nw) {
+    if (lane < R) {
+      T acc = (T)0;
+      for (int64_t j = a.seg_row_ptr[i]; j < a.seg_row_ptr[i + 1]; ++j) acc += ((const T*)a.partial)[j * R + lane];
       if (w) acc *= w[lane];
       ((T*)a.out)[i * R + lane] = acc;
     }
@@ -640,17 +665,24 @@ extern "C" int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int r
   return TLA_OK;
 }
 
+extern "C" size_t tla_mttkrp_coo_workspace_size(int dtype, int64_t n_seg, int rank) {
+  return (size_t)(n_seg > 0 ? n_seg : 1) * (size_t)rank * (dtype == TLA_F64 ? 8 : 4);
+}
+
 extern "C" int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* shape, const int64_t* coords,
-                              const void* values, const int64_t* perm, const int64_t* row_ptr,
-                              const void* const* factors, const void* weights, int rank, int mode, void* out,
-                              void* stream) {
+                              const void* values, const int64_t* perm, const int64_t* row_ptr, int64_t n_seg,
+                              const int64_t* seg_row, const int64_t* seg_start, const int64_t* seg_row_ptr,
+                              int64_t seg_len, const void* const* factors, const void* weights, int rank, int mode,
+                              void* out, void* workspace, size_t workspace_bytes, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "mttkrp_coo: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "mttkrp_coo: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "mttkrp_coo: mode %d", mode);
   TLA_REQUIRE(rank >= 1 && rank <= 64, "mttkrp_coo: rank %d not in [1, 64]", rank);
-  TLA_REQUIRE(nnz >= 0, "mttkrp_coo: nnz = %lld", (long long)nnz);
-  TLA_REQUIRE(shape && out && row_ptr, "mttkrp_coo: NULL pointer");
-  TLA_REQUIRE(nnz == 0 || (coords && values && perm), "mttkrp_coo: NULL nonzero arrays");
+  TLA_REQUIRE(nnz >= 0 && n_seg >= 0 && seg_len >= 1, "mttkrp_coo: nnz / segments");
+  TLA_REQUIRE(shape && out && row_ptr && seg_row_ptr, "mttkrp_coo: NULL pointer");
+  TLA_REQUIRE(n_seg == 0 || (coords && values && perm && seg_row && seg_start && workspace),
+              "mttkrp_coo: NULL nonzero / segment arrays");
+  TLA_REQUIRE(workspace_bytes >= tla_mttkrp_coo_workspace_size(dtype, n_seg, rank), "mttkrp_coo: workspace too small");
   CooArgs a{};
   for (int m = 0; m < ndim; ++m) {
     TLA_REQUIRE(shape[m] >= 1, "mttkrp_coo: shape[%d] = %lld", m, (long long)shape[m]);
@@ -658,11 +690,19 @@ extern "C" int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* s
     a.factors[m] = factors[m];
   }
   a.coords = coords; a.nnz = nnz; a.vals = values; a.perm = perm; a.row_ptr = row_ptr;
-  a.weights = weights; a.ndim = ndim; a.mode = mode; a.R = rank; a.rows = shape[mode]; a.out = out;
-  const int64_t blocks = std::min<int64_t>((a.rows + 3) / 4, 4096);
+  a.seg_row = seg_row; a.seg_start = seg_start; a.seg_row_ptr = seg_row_ptr; a.n_seg = n_seg; a.seg_len = seg_len;
+  a.weights = weights; a.ndim = ndim; a.mode = mode; a.R = rank; a.rows = shape[mode];
+  a.partial = workspace; a.out = out;
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == TLA_F32) mttkrp_coo_kernel<float><<<(unsigned)blocks, 256, 0, st>>>(a);
-  else mttkrp_coo_kernel<double><<<(unsigned)blocks, 256, 0, st>>>(a);
+  if (n_seg > 0) {
+    const int64_t blocks = std::min<int64_t>((n_seg + 3) / 4, 8192);
+    if (dtype == TLA_F32) mttkrp_coo_segments_kernel<float><<<(unsigned)blocks, 256, 0, st>>>(a);
+    else mttkrp_coo_segments_kernel<double><<<(unsigned)blocks, 256, 0, st>>>(a);
+    TLA_LAUNCH_CHECK();
+  }
+  const int64_t rblocks = std::min<int64_t>((a.rows + 3) / 4, 8192);
+  if (dtype == TLA_F32) mttkrp_coo_rows_kernel<float><<<(unsigned)rblocks, 256, 0, st>>>(a);
+  else mttkrp_coo_rows_kernel<double><<<(unsigned)rblocks, 256, 0, st>>>(a);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -149,6 +149,9 @@ def unfolding_dot_khatri_rao(tensor, cp_tensor, mode):
     return out
 
 
+_COO_SEG = 1024  # nonzeros per piece of a row in the COO MTTKRP
+
+
 def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
     """MTTKRP of a sparse tensor given in coordinate form: ``coords`` (ndim x
     nnz int64 GPU tensor), ``values`` (nnz) and ``shape``.  Same result as
@@ -174,18 +177,32 @@ def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
             raise ValueError(f"factor {m} has shape {tuple(factors[m].shape)}, expected ({int(s)}, {rank})")
     if nnz and (int(coords.min()) < 0 or any(int(coords[m].max()) >= int(s) for m, s in enumerate(shape))):
         raise ValueError("coordinates out of the tensor's shape")
+    dev = values.device
+    rows = int(shape[mode])
     idx = coords[mode]
-    perm = torch.argsort(idx, stable=True) if nnz else torch.zeros(0, dtype=torch.int64, device=values.device)
-    row_ptr = torch.zeros(int(shape[mode]) + 1, dtype=torch.int64, device=values.device)
-    if nnz:
-        row_ptr[1:] = torch.cumsum(torch.bincount(idx, minlength=int(shape[mode])), 0)
+    perm = torch.argsort(idx, stable=True) if nnz else torch.zeros(0, dtype=torch.int64, device=dev)
+    counts = torch.bincount(idx, minlength=rows) if nnz else torch.zeros(rows, dtype=torch.int64, device=dev)
+    row_ptr = torch.zeros(rows + 1, dtype=torch.int64, device=dev)
+    row_ptr[1:] = torch.cumsum(counts, 0)
+    # each row's nonzeros in pieces of at most _COO_SEG, so that modes with
+    # few rows still spread over many waves
+    nseg = (counts + (_COO_SEG - 1)) // _COO_SEG
+    seg_row_ptr = torch.zeros(rows + 1, dtype=torch.int64, device=dev)
+    seg_row_ptr[1:] = torch.cumsum(nseg, 0)
+    n_seg = int(seg_row_ptr[-1])
+    seg_row = torch.repeat_interleave(torch.arange(rows, device=dev), nseg)
+    seg_start = row_ptr[seg_row] + (torch.arange(n_seg, device=dev) - seg_row_ptr[seg_row]) * _COO_SEG
     facs = [None if m == mode else factors[m].to(dt).contiguous() for m in range(len(shape))]
     w = None if weights is None else weights.to(dt).contiguous()
-    out = torch.empty((int(shape[mode]), rank), dtype=dt, device=values.device)
+    out = torch.empty((rows, rank), dtype=dt, device=dev)
+    code = _lib.dtype_code(values)
+    nb = lib().tla_mttkrp_coo_workspace_size(code, n_seg, rank)
+    ws = workspace(nb, dev)
     sh, _keep_sh = shape_array(shape)
     pp, _keep_f = ptr_array(facs)
-    check(lib().tla_mttkrp_coo(_lib.dtype_code(values), nnz, len(shape), sh, ptr(coords), ptr(values.contiguous()),
-                               ptr(perm), ptr(row_ptr), pp, ptr(w), rank, mode, ptr(out), stream(values.device)),
+    check(lib().tla_mttkrp_coo(code, nnz, len(shape), sh, ptr(coords), ptr(values.contiguous()), ptr(perm),
+                               ptr(row_ptr), n_seg, ptr(seg_row), ptr(seg_start), ptr(seg_row_ptr), _COO_SEG, pp,
+                               ptr(w), rank, mode, ptr(out), ptr(ws), nb, stream(dev)),
           "mttkrp_coo")
     return out
 
