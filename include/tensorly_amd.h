@@ -225,12 +225,16 @@ int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void* mttkrp, vo
  * at most n_iter_max iterations, stopping when an iteration's squared update
  * is below tol times the first iteration's.  factor (rows x rank, dtype) is
  * updated in place, in FP64 arithmetic; iters_out (device int, nullable)
- * receives the iterations run.  Arguments otherwise as tla_nn_mu_update.
+ * receives the iterations run.  With a workspace (nullable) and
+ * n_iter_max <= 1000 each iteration is one launch over the whole GPU (a
+ * device-side flag ends the remaining launches early); otherwise one
+ * workgroup runs the loop.  Arguments otherwise as tla_nn_mu_update.
  * ------------------------------------------------------------------- */
+size_t tla_hals_nnls_workspace_size(int64_t rows, int rank);
 int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
                   const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
                   double sparsity, int has_sparsity, int n_iter_max, double tol, int* iters_out,
-                  void* stream);
+                  void* workspace, size_t workspace_bytes, void* stream);
 
 
 /* ---------------------------------------------------------------------

@@ -54,7 +54,9 @@ _SIGNATURES = {
     "tla_cp_error": (c_i, [PP, c_i, c_p, c_i, c_i, c_p, c_p, c_p, c_i, c_p]),
     "tla_nn_mu_update": (c_i, [c_i, c_p, c_p, c_i64, c_i, PP, c_i, c_i, c_p, c_i, c_d, c_p]),
     "tla_cp_lstsq_grad_mode": (c_i, [c_i, c_p, c_p, c_p, c_i64, c_i, PP, c_i, c_i, c_p, c_i, c_p]),
-    "tla_hals_nnls": (c_i, [c_i, c_p, c_p, c_i64, c_i, PP, c_i, c_i, c_p, c_i, c_d, c_i, c_i, c_d, c_p, c_p]),
+    "tla_hals_nnls_workspace_size": (c_sz, [c_i64, c_i]),
+    "tla_hals_nnls": (c_i, [c_i, c_p, c_p, c_i64, c_i, PP, c_i, c_i, c_p, c_i, c_d, c_i, c_i, c_d, c_p, c_p, c_sz,
+                            c_p]),
     "tla_mttkrp_coo_workspace_size": (c_sz, [c_i, c_i64, c_i]),
     "tla_mttkrp_coo": (c_i, [c_i, c_i64, c_i, P_i64, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_i64, PP, c_p, c_i,
                              c_i, c_p, c_p, c_sz, c_p]),

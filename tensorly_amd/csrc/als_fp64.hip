@@ -475,6 +475,87 @@ __global__ __launch_bounds__(kHalsThreads) void hals_nnls_kernel(HalsArgs a) {
   if (tid == 0 && a.iters_out) *a.iters_out = it;
 }
 
+
+// Multi-workgroup HALS: one launch per iteration (rows over the grid, the
+// workgroup's squared update summed in a fixed order into partial[block]),
+// then one single-thread launch that sums the partials in block order and
+// sets the stop flag; later iteration launches see the flag and return.
+// state: [0] stop flag, [1] iterations run; err0 at state_d[0].
+template <int RT>
+__global__ void hals_prep_kernel(HalsArgs a, double* utu_g, int* state, double* err0) {
+  __shared__ double acc[kMaxRank * kMaxRank];
+  build_v(a, a.ndim, a.mode, -1, nullptr, a.weights, a.wdt, 0.0, a.R, acc);
+  for (int e = threadIdx.x; e < RT * RT; e += blockDim.x) {
+    const int k = e / RT, l = e % RT;
+    utu_g[e] = (k < a.R && l < a.R) ? acc[k * a.R + l] : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    state[0] = 0;
+    state[1] = 0;
+    err0[0] = 0.0;
+  }
+}
+
+constexpr int kHalsRowsPerBlock = 64;
+
+template <typename T, int RT>
+__global__ __launch_bounds__(kHalsRowsPerBlock) void hals_iter_kernel(HalsArgs a, const double* utu_g,
+                                                                      double* partial, const int* state) {
+  if (state[0]) return;
+  __shared__ double utu[RT * RT];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < RT * RT; e += blockDim.x) utu[e] = utu_g[e];
+  __syncthreads();
+  const int R = a.R;
+  T* A = (T*)a.A;
+  const T* M = (const T*)a.M;
+  double err = 0.0;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  if (i < a.rows) {
+    const double* U = utu;
+    asm volatile("" : "+v"(U));
+    constexpr bool kMRegs = RT <= 48;
+    double v[RT], m[kMRegs ? RT : 1];
+#pragma unroll
+    for (int l = 0; l < RT; ++l) {
+      v[l] = l < R ? (double)A[i * R + l] : 0.0;
+      if constexpr (kMRegs) m[l] = l < R ? (double)M[i * R + l] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < RT; ++k) {
+      const double d = U[k * RT + k];
+      if (d != 0.0) {
+        double dot = 0.0;
+#pragma unroll
+        for (int l = 0; l < RT; ++l) dot = fma(U[k * RT + l], v[l], dot);
+        double mk;
+        if constexpr (kMRegs) mk = m[k];
+        else mk = (double)M[i * R + k];
+        double num = mk - dot + d * v[k];
+        if (a.has_sparsity) num -= a.sparsity;
+        const double nv = fmax(num / d, 0.0);
+        const double diff = v[k] - nv;
+        err = fma(diff, diff, err);
+        v[k] = nv;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < RT; ++l)
+      if (l < R) A[i * R + l] = (T)v[l];
+  }
+  err = wave_sum(err);
+  if (tid == 0) partial[blockIdx.x] = err;
+}
+
+__global__ void hals_decide_kernel(const double* partial, int nblocks, double tol, int* state, double* err0) {
+  if (threadIdx.x != 0 || state[0]) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partial[b];
+  if (state[1] == 0) err0[0] = s;
+  state[1] += 1;
+  if (s < tol * err0[0]) state[0] = 1;
+}
+
 }  // namespace tla
 
 using namespace tla;
@@ -660,10 +741,22 @@ extern "C" int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void*
   return TLA_OK;
 }
 
+static size_t hals_align(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Iteration counts up to this run as one launch per iteration over the grid;
+// longer runs (exact mode) stay in one workgroup.
+constexpr int kHalsMaxLaunches = 1000;
+
+extern "C" size_t tla_hals_nnls_workspace_size(int64_t rows, int rank) {
+  (void)rank;
+  const int64_t nb = (rows + kHalsRowsPerBlock - 1) / kHalsRowsPerBlock;
+  return hals_align(64 * 64 * 8) + hals_align(size_t(nb > 0 ? nb : 1) * 8) + hals_align(16) + hals_align(8);
+}
+
 extern "C" int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,
                              const double* const* grams, int ndim, int mode, const void* weights, int wdtype,
                              double sparsity, int has_sparsity, int n_iter_max, double tol, int* iters_out,
-                             void* stream) {
+                             void* workspace, size_t workspace_bytes, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "hals_nnls: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "hals_nnls: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "hals_nnls: mode %d", mode);
@@ -685,6 +778,43 @@ extern "C" int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_
   a.n_iter_max = n_iter_max; a.tol = tol; a.iters_out = iters_out;
   hipStream_t st = (hipStream_t)stream;
   const int rt = rank <= 16 ? 16 : rank <= 32 ? 32 : rank <= 48 ? 48 : 64;
+  if (workspace && n_iter_max <= kHalsMaxLaunches) {
+    TLA_REQUIRE(workspace_bytes >= tla_hals_nnls_workspace_size(rows, rank), "hals_nnls: workspace too small");
+    const int64_t nb = (rows + kHalsRowsPerBlock - 1) / kHalsRowsPerBlock;
+    TLA_REQUIRE(nb <= (int64_t)1 << 30, "hals_nnls: %lld rows", (long long)rows);
+    char* w = (char*)workspace;
+    double* utu_g = (double*)w;
+    double* partial = (double*)(w + hals_align(64 * 64 * 8));
+    int* state = (int*)(w + hals_align(64 * 64 * 8) + hals_align(size_t(nb) * 8));
+    double* err0 = (double*)((char*)state + hals_align(16));
+#define TLA_HALS_MULTI(T_, RT_)                                                                       \
+  do {                                                                                                \
+    hals_prep_kernel<RT_><<<1, 256, 0, st>>>(a, utu_g, state, err0);                                  \
+    for (int it = 0; it < n_iter_max; ++it) {                                                         \
+      hals_iter_kernel<T_, RT_><<<(unsigned)nb, kHalsRowsPerBlock, 0, st>>>(a, utu_g, partial, state); \
+      hals_decide_kernel<<<1, 64, 0, st>>>(partial, (int)nb, tol, state, err0);                       \
+    }                                                                                                 \
+  } while (0)
+#define TLA_HALS_RT(T_)                                \
+  switch (rt) {                                        \
+    case 16: TLA_HALS_MULTI(T_, 16); break;            \
+    case 32: TLA_HALS_MULTI(T_, 32); break;            \
+    case 48: TLA_HALS_MULTI(T_, 48); break;            \
+    default: TLA_HALS_MULTI(T_, 64); break;            \
+  }
+    if (dtype == TLA_F32) {
+      TLA_HALS_RT(float)
+    } else {
+      TLA_HALS_RT(double)
+    }
+#undef TLA_HALS_RT
+#undef TLA_HALS_MULTI
+    TLA_LAUNCH_CHECK();
+    if (iters_out) {
+      TLA_HIP(hipMemcpyAsync(iters_out, state + 1, sizeof(int), hipMemcpyDeviceToDevice, st));
+    }
+    return TLA_OK;
+  }
 #define TLA_HALS(T_)                                                                  \
   switch (rt) {                                                                       \
     case 16: hals_nnls_kernel<T_, 16><<<1, kHalsThreads, 0, st>>>(a); break;          \

@@ -121,9 +121,12 @@ def _hals_update(factors, mttkrp, grams, weights, mode, sparsity, n_iter_max, to
     rows, rank = f.shape
     m = mttkrp.contiguous()
     gp, _keep = ptr_array(grams)
+    nb = lib().tla_hals_nnls_workspace_size(rows, rank)
+    ws = _lib.workspace(nb, f.device)
     check(lib().tla_hals_nnls(_lib.dtype_code(f), ptr(f), ptr(m), rows, rank, gp, len(factors), mode, ptr(weights),
                               _lib.dtype_code(weights), 0.0 if sparsity is None else float(sparsity),
-                              0 if sparsity is None else 1, int(n_iter_max), float(tol), None, stream(f.device)),
+                              0 if sparsity is None else 1, int(n_iter_max), float(tol), None, ptr(ws), nb,
+                              stream(f.device)),
           "hals_nnls")
 
 
