@@ -56,7 +56,9 @@ int tla_abi_version(void);
  *   x_dtype  TLA_F32 / TLA_F64 / TLA_BF16; factors, weights and out are F64
  *            when x_dtype is F64, else F32.
  *   factors  host array of ndim device pointers (entry `mode` is ignored).
- *   weights  device pointer of R values, or NULL for ones.
+ *   weights  device pointer of R values, or NULL for ones.  Ignored when
+ *            ndim == 2: khatri_rao of the one remaining matrix returns it
+ *            unweighted (tensorly/tenalg/core_tenalg/_khatri_rao.py:67-69).
  * ------------------------------------------------------------------- */
 size_t tla_mttkrp_workspace_size(int x_dtype, int ndim, const int64_t* shape, int rank,
                                  int mode, int precision);
@@ -108,7 +110,8 @@ int tla_rankdiag_contract(const float* Y, int64_t I, int64_t J, int rank, const 
  *            row and first position in perm; seg_row_ptr (shape[mode] + 1)
  *            delimits each row's pieces;
  *   factors  host array of ndim device pointers (entry `mode` unused), dtype;
- *   weights  R values (dtype) or NULL.  rank <= 64.
+ *   weights  R values (dtype) or NULL (ignored when ndim == 2, as for
+ *            tla_mttkrp).  rank <= 64.
  *   workspace  tla_mttkrp_coo_workspace_size(dtype, n_seg, rank) bytes.
  * ------------------------------------------------------------------- */
 size_t tla_mttkrp_coo_workspace_size(int dtype, int64_t n_seg, int rank);
@@ -222,8 +225,10 @@ int tla_cp_lstsq_grad_mode(int dtype, const void* factor, const void* mttkrp, vo
  * V = factor^T are updated in order,
  *   V[k,:] = max((UtM[k,:] - UtU[k,:] V + UtU[k,k] V[k,:] - s) / UtU[k,k], 0)
  * (s = sparsity when has_sparsity; rows with UtU[k,k] == 0 are skipped), for
- * at most n_iter_max iterations, stopping when an iteration's squared update
- * is below tol times the first iteration's.  factor (rows x rank, dtype) is
+ * at most n_iter_max iterations, stopping when an iteration's error sum is
+ * below tol times the first iteration's; each row update adds
+ * ||V - newV||^2 with the new row broadcast against every row of the current
+ * V, as the reference does (tensorly/solvers/nnls.py:147).  factor (rows x rank, dtype) is
  * updated in place, in FP64 arithmetic; iters_out (device int, nullable)
  * receives the iterations run.  With a workspace (nullable) and
  * n_iter_max <= 1000 each iteration is one launch over the whole GPU (a
@@ -288,7 +293,9 @@ int tla_cp_als_set_normx2(void* handle, const double* normx2, void* stream);
 int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int first_slot, int normalize,
                       void* stream);
 /* copy state to a device buffer: 0 Grams (ndim x R x R FP64), 1 ‖X‖²,
- * 2 last <M, A> (FP64), 3 last solve status (int) */
+ * 2 last <M, A> (FP64), 3 last solve status (int), 4 largest solve status
+ * since the last refresh (int; 2 = a singular normal-equations matrix was
+ * met, where tensorly/decomposition/_cp.py:425-427's tl.solve raises) */
 int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t bytes, void* stream);
 void tla_cp_als_destroy(void* handle);
 

@@ -194,14 +194,17 @@ def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
     """MTTKRP of a sparse tensor in coordinate form (``contrib/sparse/
     cp_tensor.py:9``: ``unfolding_dot_khatri_rao`` over a sparse backend):
     each nonzero adds ``val * w * prod_{m != mode} A_m[i_m, :]`` to row
-    ``i_mode`` (repeated coordinates add up, as when densified)."""
+    ``i_mode`` (repeated coordinates add up, as when densified).  At order 2
+    the weights are not applied: the sparse backend wraps the core
+    ``unfolding_dot_khatri_rao`` (``contrib/sparse/cp_tensor.py:9``), whose
+    Khatri-Rao product of a single matrix drops them (``_khatri_rao.py:67-69``)."""
     weights, factors = cp_tensor
     rank = factors[0].shape[1]
     contrib = np.asarray(values, dtype=np.float64)[:, None] * np.ones((1, rank))
     for m in range(len(shape)):
         if m != mode:
             contrib = contrib * factors[m][coords[m]]
-    if weights is not None:
+    if weights is not None and len(shape) > 2:
         contrib = contrib * weights[None, :]
     out = np.zeros((shape[mode], rank))
     np.add.at(out, coords[mode], contrib)
@@ -722,8 +725,9 @@ def hals_nnls(UtM, UtU, V=None, n_iter_max=500, tol=1e-8, sparsity_coefficient=N
     """``solvers/nnls.py:5-173``: block-coordinate (HALS) descent for
     min_{V >= 0} ||M - U V||^2 given UtM = U^T M (rank x n) and UtU = U^T U;
     rows of V are updated in order, each with the rows before it already
-    updated, until the squared update falls below ``tol`` times the first
-    iteration's."""
+    updated, until the iteration's sum of ||V - newV||^2 (the new row
+    broadcast against every row of the current V, ``nnls.py:147``) falls
+    below ``tol`` times the first iteration's."""
     rank = UtM.shape[0]
     if V is None:
         V = np.linalg.solve(UtU, UtM)
@@ -746,7 +750,10 @@ def hals_nnls(UtM, UtU, V=None, n_iter_max=500, tol=1e-8, sparsity_coefficient=N
                 if ridge_coefficient is not None:
                     den = den + 2 * ridge_coefficient
                 newV = np.clip(num / den, epsilon, None)
-                rec_error += np.linalg.norm(V[k, :] - newV) ** 2
+                # the reference subtracts the new row from every row of V
+                # (broadcast), not from row k alone: the stopping rule sums
+                # ||V[j, :] - newV||^2 over all rows j
+                rec_error += np.linalg.norm(V - newV) ** 2
                 V[k, :] = newV
                 if nonzero_rows and np.all(V[k, :] == 0):
                     V[k, :] = np.finfo(V.dtype).eps * np.max(V)
@@ -942,3 +949,335 @@ def tucker(tensor, rank, fixed_factors=None, n_iter_max=100, init="svd", return_
     if return_errors:
         return (core, factors), errs
     return (core, factors)
+
+
+# --------------------------------------------------------------------------
+# Constrained CP by AO-ADMM  (decomposition/_constrained_cp.py,
+# solvers/admm.py, tenalg/proximal.py)
+# --------------------------------------------------------------------------
+CONSTRAINT_NAMES = ("non_negative", "l1_reg", "l2_reg", "l2_square_reg", "unimodality", "normalize", "simplex",
+                    "normalized_sparsity", "soft_sparsity", "smoothness", "monotonicity", "hard_sparsity")
+
+
+def validate_constraints(n_const=1, order=0, **constraints):
+    """``tenalg/proximal.py:4-150``: the constraint and parameter of mode
+    ``order``.  Each keyword is None/False (unused), a scalar or True (every
+    mode), a list (per mode, falsy entries unused) or a dict (mode ->
+    parameter); a mode constrained twice raises ValueError.  Later names in
+    the reference's order overwrite earlier ones for the modes they set."""
+    unknown = set(constraints) - set(CONSTRAINT_NAMES)
+    if unknown:
+        raise TypeError(f"unknown constraints {sorted(unknown)}")
+    taken = set()
+    for name in CONSTRAINT_NAMES:
+        spec = constraints.get(name)
+        if not spec:
+            continue
+        if isinstance(spec, dict):
+            modes = list(spec)
+        elif isinstance(spec, list):
+            modes = [m for m in range(len(spec)) if spec[m]]
+        else:
+            if taken:
+                raise ValueError("You selected two constraints for the same mode. Consider to check your input")
+            modes = list(range(n_const))
+        for m in modes:
+            if m in taken:
+                raise ValueError("You selected two constraints for the same mode. Consider to check your input")
+            taken.add(m)
+    kinds = [None] * n_const
+    params = [None] * n_const
+    for name in CONSTRAINT_NAMES:
+        spec = constraints.get(name)
+        if not spec:
+            continue
+        if isinstance(spec, dict):
+            for m, p in spec.items():
+                kinds[m], params[m] = name, p
+        else:
+            for m in range(n_const):
+                kinds[m] = name
+                params[m] = spec[m] if isinstance(spec, list) else spec
+    return kinds[order], params[order]
+
+
+def soft_thresholding(tensor, threshold):
+    """``tenalg/proximal.py:659-689``: sign(x) max(|x| - t, 0)."""
+    return np.sign(tensor) * np.clip(np.abs(tensor) - threshold, 0, None)
+
+
+def l2_square_prox(tensor, regularizer):
+    """``tenalg/proximal.py:478-500``."""
+    return tensor / (1 + 2 * regularizer)
+
+
+def l2_prox(tensor, regularizer):
+    """``tenalg/proximal.py:503-529``: block soft thresholding with the
+    Frobenius norm of the whole matrix."""
+    n = np.linalg.norm(tensor)
+    big = n if n > regularizer else regularizer
+    return tensor - tensor * regularizer / big
+
+
+def hard_thresholding(tensor, number_of_non_zero):
+    """``tenalg/proximal.py:634-656``: keep the ``number_of_non_zero``
+    entries of largest magnitude (ranks from a double argsort of |x|, stable
+    here so that equal magnitudes rank the later flat index first)."""
+    v = np.ravel(tensor)
+    order = np.argsort(np.abs(v), kind="stable")[::-1]
+    ranks = np.empty(v.shape[0], dtype=np.int64)
+    ranks[order] = np.arange(v.shape[0])
+    return np.reshape(np.where(ranks < number_of_non_zero, v, 0.0), np.shape(tensor))
+
+
+def normalized_sparsity_prox(tensor, threshold):
+    """``tenalg/proximal.py:532-561``."""
+    h = hard_thresholding(tensor, threshold)
+    return h / np.linalg.norm(h)
+
+
+def simplex_prox(tensor, parameter):
+    """``tenalg/proximal.py:592-631``: each column projected on the simplex
+    {x >= 0, sum x = parameter} by the sorted-cumsum rule (Held, Wolfe and
+    Crowder): c_k = (sum of the k largest - parameter) / k, K = #{k: u_k >
+    c_k}, x = max(v - c_K, 0)."""
+    t = np.asarray(tensor, dtype=np.float64)
+    one_d = t.ndim == 1
+    if one_d:
+        t = t[:, None]
+    u = -np.sort(-t, axis=0)
+    c = (np.cumsum(u, axis=0) - parameter) / np.arange(1, t.shape[0] + 1)[:, None]
+    K = np.sum(u > c, axis=0) - 1
+    shift = c[K, np.arange(t.shape[1])]
+    out = np.clip(t - shift[None, :], 0, None)
+    return out[:, 0] if one_d else out
+
+
+def soft_sparsity_prox(tensor, threshold):
+    """``tenalg/proximal.py:564-589``."""
+    return simplex_prox(np.abs(tensor), threshold) * np.sign(tensor)
+
+
+def smoothness_prox(tensor, regularizer):
+    """``tenalg/proximal.py:270-291``: solve with the tridiagonal matrix
+    (2 lambda + 1 on the diagonal, -lambda beside it)."""
+    n = tensor.shape[0]
+    A = (np.diag(np.full(n, 2 * regularizer + 1.0)) + np.diag(np.full(n - 1, -regularizer), 1)
+         + np.diag(np.full(n - 1, -regularizer), -1))
+    return np.linalg.solve(A, tensor)
+
+
+def monotonicity_prox(tensor, decreasing=False):
+    """``tenalg/proximal.py:294-356``, per column: y_j = the largest mean of
+    x[i..j] over i <= j, floored at 0 for every j but the last (the
+    reference's helper matrix holds zeros below its diagonal), then each y_j
+    lowered to min(y_j, y_{j+1}) from the bottom up; ``decreasing`` works on
+    the flipped column."""
+    t = np.array(tensor, dtype=np.float64)
+    if t.ndim == 1:
+        t = t[:, None]
+    elif t.ndim > 2:
+        raise ValueError("Monotonicity prox doesn't support an input which has more than 2 dimensions.")
+    if decreasing:
+        t = t[::-1]
+    n, ncol = t.shape
+    cum = np.cumsum(t, axis=0)
+    out = t.copy()
+    for c in range(ncol):
+        helper = np.zeros((n, n))
+        for i in range(n):
+            prev = cum[i - 1, c] if i > 0 else 0.0
+            helper[i, i:] = (cum[i:, c] - prev) / (np.arange(n - i) + 1)
+        col = np.max(helper, axis=0)
+        for i in range(n - 2, -1, -1):
+            if col[i] > col[i + 1]:
+                col[i] = col[i + 1]
+        out[:, c] = col
+    if decreasing:
+        out = out[::-1]
+    return out
+
+
+def unimodality_prox(tensor):
+    """``tenalg/proximal.py:359-475``: per column, the peak j among entries
+    at or above both monotone projections minimising sum_{k<j}|x_k - inc_k|
+    + sum_{k>j}|x_k - dec_k| (columns with no such entry, or whose best cost
+    is the global maximum, take j = 0); the result is inc above the peak,
+    x at it and dec below it."""
+    t = np.array(tensor, dtype=np.float64)
+    if t.ndim == 1:
+        t = t[:, None]
+    elif t.ndim > 2:
+        raise ValueError("Unimodality prox doesn't support an input which has more than 2 dimensions.")
+    inc = monotonicity_prox(t)
+    dec = monotonicity_prox(t, decreasing=True)
+    peak = (t - dec >= 0) & (t - inc >= 0)
+    a_inc = np.abs(t - inc)
+    a_dec = np.abs(t - dec)
+    before = np.where(peak, np.cumsum(a_inc, axis=0) - a_inc, 0.0)
+    after_flipped = np.cumsum(a_dec[::-1], axis=0) - a_dec[::-1]
+    after = np.where(peak, after_flipped[::-1], 0.0)
+    cost = before + after
+    cost = np.where(peak, cost, np.max(cost))
+    jstar = np.argmin(cost, axis=0)
+    out = t.copy()
+    for c, j in enumerate(jstar):
+        out[:j, c] = inc[:j, c]
+        out[j + 1:, c] = dec[j + 1:, c]
+    return out
+
+
+def proximal_operator(tensor, n_const=1, order=0, **constraints):
+    """``tenalg/proximal.py:153-267``."""
+    if n_const is None:
+        return tensor
+    kind, p = validate_constraints(n_const=n_const, order=order, **constraints)
+    if kind is None:
+        return tensor
+    if kind == "non_negative":
+        # np.clip(x, 0, max(x)): when every entry is negative the bounds
+        # cross and numpy returns the upper bound everywhere
+        return np.minimum(np.maximum(tensor, 0), np.max(tensor))
+    if kind == "l1_reg":
+        return soft_thresholding(tensor, p)
+    if kind == "l2_reg":
+        return l2_prox(tensor, p)
+    if kind == "l2_square_reg":
+        return l2_square_prox(tensor, p)
+    if kind == "unimodality":
+        return unimodality_prox(tensor)
+    if kind == "normalize":
+        return tensor / np.max(np.abs(tensor))
+    if kind == "simplex":
+        return simplex_prox(tensor, p)
+    if kind == "normalized_sparsity":
+        return normalized_sparsity_prox(tensor, p)
+    if kind == "soft_sparsity":
+        return soft_sparsity_prox(tensor, p)
+    if kind == "smoothness":
+        return smoothness_prox(tensor, p)
+    if kind == "monotonicity":
+        return monotonicity_prox(tensor)
+    if kind == "hard_sparsity":
+        return hard_thresholding(tensor, p)
+    raise RuntimeError("Invalid constraint name")
+
+
+def admm(UtM, UtU, x, dual_var, n_iter_max=100, n_const=None, order=None, tol=1e-4, **constraints):
+    """``solvers/admm.py:5-155``.  With rho = trace(UtU) / R:
+    x_split = solve((UtU + rho I)^T, (UtM + rho (x + dual))^T),
+    x = prox(x_split^T - dual), dual += x - x_split^T, stopping when
+    ||x - x_split^T|| < tol ||x|| and ||x - x_old|| < tol ||dual||.
+    Returns (x, x_split, dual)."""
+    R = x.shape[1]
+    rho = np.trace(UtU) / R
+    x_split = None
+    for _ in range(n_iter_max):
+        x_old = x.copy()
+        x_split = np.linalg.solve(np.transpose(UtU + rho * np.eye(UtU.shape[1])),
+                                  np.transpose(UtM + rho * (x + dual_var)))
+        x = proximal_operator(np.transpose(x_split) - dual_var, n_const=n_const, order=order, **constraints)
+        if n_const is None:
+            x = np.transpose(np.linalg.solve(np.transpose(UtU), np.transpose(UtM)))
+            return x, x_split, dual_var
+        dual_var = dual_var + x - np.transpose(x_split)
+        dual_residual = x - np.transpose(x_split)
+        primal_residual = x - x_old
+        if (np.linalg.norm(dual_residual) < tol * np.linalg.norm(x)
+                and np.linalg.norm(primal_residual) < tol * np.linalg.norm(dual_var)):
+            break
+    return x, x_split, dual_var
+
+
+def initialize_constrained_parafac(tensor, rank, init="svd", svd="truncated_svd", random_state=None,
+                                   **constraints):
+    """``decomposition/_constrained_cp.py:22-163``.  ``init='random'`` draws
+    from the global numpy RandomState (the reference passes no
+    ``random_state`` to ``random_cp``, ``:104-107``); every factor then goes
+    through its mode's proximal operator.  A CP tensor init returns its
+    factors scaled by the weights' geometric mean, unprojected."""
+    n_modes = tensor.ndim
+    rng = check_random_state(random_state)
+    if isinstance(init, str) and init == "random":
+        _, factors = random_cp(tensor.shape, rank, normalise_factors=False, random_state=None,
+                               dtype=tensor.dtype)
+    elif isinstance(init, str) and init == "svd":
+        factors = []
+        for mode in range(n_modes):
+            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd)
+            if mode == 0:
+                k = min(rank, S.shape[0])
+                U = U.copy()
+                U[:, :k] = U[:, :k] * S[:k]
+            if tensor.shape[mode] < rank:
+                pad = np.asarray(rng.random_sample((U.shape[0], rank - tensor.shape[mode])), dtype=tensor.dtype)
+                U = np.concatenate([U, pad], axis=1)
+            factors.append(U[:, :rank])
+    elif isinstance(init, (tuple, list)):
+        weights, factors = init
+        factors = [np.array(f) for f in factors]
+        if weights is None or np.all(weights == 1):
+            return np.ones(rank, dtype=factors[0].dtype), factors
+        avg = np.prod(weights) ** (1.0 / weights.shape[0])
+        return np.ones(rank, dtype=factors[0].dtype), [f * avg for f in factors]
+    else:
+        raise ValueError(f'Initialization method "{init}" not recognized')
+    factors = [proximal_operator(f, n_const=n_modes, order=i, **constraints) for i, f in enumerate(factors)]
+    return np.ones(rank, dtype=tensor.dtype), factors
+
+
+def constrained_parafac(tensor, rank, n_iter_max=100, n_iter_max_inner=10, init="svd", svd="truncated_svd",
+                        tol_outer=1e-8, tol_inner=1e-6, random_state=None, verbose=0, return_errors=False,
+                        cvg_criterion="abs_rec_error", fixed_modes=None, **constraints):
+    """``decomposition/_constrained_cp.py:166-424``: per mode the MTTKRP
+    with unit weights (``:352``) and one ADMM solve; the error from the last
+    mode's MTTKRP and the Grams (``:379-383``); the outer loop stops on the
+    summed constraint violation or on the error decrease (``:384-411``)."""
+    rank = validate_cp_rank(tensor.shape, rank=rank)
+    validate_constraints(n_const=tensor.ndim, **constraints)
+    weights, factors = initialize_constrained_parafac(tensor, rank, init=init, svd=svd, random_state=random_state,
+                                                      **constraints)
+    factors = list(factors)
+    rec_errors = []
+    norm_tensor = norm(tensor, 2)
+    fixed_modes = list(fixed_modes) if fixed_modes is not None else []
+    if tensor.ndim - 1 in fixed_modes:
+        warnings.warn("You asked for fixing the last mode, which is not supported.")
+        fixed_modes.remove(tensor.ndim - 1)
+    modes_list = [m for m in range(tensor.ndim) if m not in fixed_modes]
+    duals = [np.zeros(f.shape) for f in factors]
+    aux = [np.zeros(f.shape).T for f in factors]
+    mttkrp = None
+    for iteration in range(n_iter_max):
+        for mode in modes_list:
+            V = np.ones((rank, rank), dtype=tensor.dtype)
+            for i, f in enumerate(factors):
+                if i != mode:
+                    V = V * np.dot(np.transpose(f), f)
+            mttkrp = unfolding_dot_khatri_rao(tensor, (None, factors), mode)
+            factors[mode], aux[mode], duals[mode] = admm(mttkrp, V, factors[mode], duals[mode],
+                                                         n_iter_max=n_iter_max_inner, n_const=tensor.ndim,
+                                                         order=mode, tol=tol_inner, **constraints)
+        factors_norm = cp_norm((weights, factors))
+        iprod = np.sum(np.sum(mttkrp * factors[-1], axis=0) * weights)
+        rec_error = np.sqrt(np.abs(norm_tensor ** 2 + factors_norm ** 2 - 2 * iprod)) / norm_tensor
+        rec_errors.append(rec_error)
+        constraint_error = 0.0
+        for mode in modes_list:
+            constraint_error += np.linalg.norm(factors[mode] - np.transpose(aux[mode])) / np.linalg.norm(factors[mode])
+        if tol_outer:
+            if iteration >= 1:
+                dec = rec_errors[-2] - rec_errors[-1]
+                if constraint_error < tol_outer:
+                    break
+                if cvg_criterion == "abs_rec_error":
+                    stop = abs(dec) < tol_outer
+                elif cvg_criterion == "rec_error":
+                    stop = dec < tol_outer
+                else:
+                    raise TypeError("Unknown convergence criterion")
+                if stop:
+                    break
+    if return_errors:
+        return (weights, factors), rec_errors
+    return (weights, factors)

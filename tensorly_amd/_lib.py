@@ -40,6 +40,7 @@ _SIGNATURES = {
     "tla_mttkrp_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i]),
     "tla_mttkrp": (c_i, [c_i, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_i, c_p, c_p, c_sz, c_p]),
     "tla_mttkrp_partial_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i, c_i]),
+    "tla_mttkrp_partial_supported": (c_i, [c_i, c_i, P_i64, c_i, c_i, c_i, c_i]),
     "tla_mttkrp_partial": (c_i, [c_i, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_i, c_i, c_p, c_p, c_sz, c_p]),
     "tla_rankdiag_contract_workspace_size": (c_sz, [c_i64, c_i64, c_i, c_i]),
     "tla_rankdiag_contract": (c_i, [c_p, c_i64, c_i64, c_i, c_p, c_i, c_p, c_p, c_sz, c_p]),
@@ -60,6 +61,10 @@ _SIGNATURES = {
     "tla_mttkrp_coo_workspace_size": (c_sz, [c_i, c_i64, c_i]),
     "tla_mttkrp_coo": (c_i, [c_i, c_i64, c_i, P_i64, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_i64, PP, c_p, c_i,
                              c_i, c_p, c_p, c_sz, c_p]),
+    "tla_mode_gram64_workspace_size": (c_sz, [c_i, P_i64, c_i]),
+    "tla_mode_gram64": (c_i, [c_i, c_p, c_i, P_i64, c_i, c_p, c_p, c_sz, c_p]),
+    "tla_sym_eig_workspace_size": (c_sz, [c_i]),
+    "tla_sym_eig": (c_i, [c_p, c_i, c_i, c_p, c_i, c_p, c_p, c_d, c_i, c_p, c_p, c_sz, c_p]),
     "tla_cp_als_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i]),
     "tla_cp_als_dimtree_supported": (c_i, [c_i, c_i, P_i64, c_i, c_i]),
     "tla_cp_als_create": (c_i, [c_i, c_p, c_i, P_i64, c_i, PP, c_p, c_d, c_i, c_i, ctypes.POINTER(c_i), c_p,
@@ -128,6 +133,15 @@ def require_gpu_tensor(t, name="tensor"):
                         "tensorly_amd has no CPU path")
     if t.is_complex():
         raise NotImplementedError("tensorly_amd kernels are real-valued only")
+
+
+def require_on_device(t, device, name):
+    """``t`` must be a GPU tensor on ``device`` (the device of the call's
+    main operand): a pointer from another device or from host memory must
+    never reach a kernel."""
+    require_gpu_tensor(t, name)
+    if t.device != device:
+        raise TypeError(f"{name} is on {t.device}, but the operation runs on {device}")
 
 
 def dtype_code(t):

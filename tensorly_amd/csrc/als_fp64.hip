@@ -104,6 +104,7 @@ struct SolveArgs {
   double* gram_partial;  // [nblk][R][R]
   double* ip_partial;    // [nblk] or nullptr
   int* status;           // nullable
+  int* status_max;       // nullable: keeps the largest status of every solve since it was zeroed
   int* counter;          // nullable: zero-initialised block counter -> the last block finalises
   double* gram_out;
   double* iprod_out;
@@ -161,7 +162,10 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   } else {
     status = invert_v<256>(a, a.ndim, a.mode, -1, nullptr, a.weights, wdt, a.l2, R, Lbuf, Vi, pbuf, pflag, &piv, &flag);
   }
-  if (blockIdx.x == 0 && tid == 0 && a.status) *a.status = status;
+  if (blockIdx.x == 0 && tid == 0) {
+    if (a.status) *a.status = status;
+    if (a.status_max && status > *a.status_max) *a.status_max = status;
+  }
   __syncthreads();
   // New rows A = M V^-1 (V symmetric) on FP64 MFMA: wave w computes rows
   // 16 w .. 16 w + 15, every 16-column tile.  The rounded values go to the
@@ -378,8 +382,10 @@ __global__ __launch_bounds__(256) void nn_mu_update_kernel(MuArgs a) {
 // UtU = (Hadamard of the other modes' Grams) * w w^T, UtM = M^T, V = A^T.
 // Each iteration updates the rows k = 0..R-1 of V in order (Gauss-Seidel),
 //   V[k,:] <- max((UtM[k,:] - UtU[k,:] V + UtU[k,k] V[k,:] - s) / UtU[k,k], 0)
-// and stops when the iteration's squared update is below tol times the
-// first iteration's.  The columns of V (rows of A) are independent within an
+// and stops when the iteration's error sum is below tol times the first
+// iteration's; as in the reference (nnls.py:147, ||V - newV||^2 with the new
+// row broadcast against all of V) each row update adds
+// sum_{j < R} (V[j, i] - newV[i])^2 over the columns i.  The columns of V (rows of A) are independent within an
 // iteration: one workgroup, each thread keeps its rows' values in registers
 // for the R updates; the squared update is summed in a fixed order.
 // ---------------------------------------------------------------------------
@@ -446,8 +452,14 @@ __global__ __launch_bounds__(kHalsThreads) void hals_nnls_kernel(HalsArgs a) {
           double num = mk - dot + d * v[k];
           if (a.has_sparsity) num -= a.sparsity;
           const double nv = fmax(num / d, 0.0);
-          const double diff = v[k] - nv;
-          err = fma(diff, diff, err);
+          // the stopping sum subtracts the new value from every entry of
+          // this column of V (the reference's broadcast ||V - newV||^2)
+#pragma unroll
+          for (int l = 0; l < RT; ++l)
+            if (l < R) {
+              const double df = v[l] - nv;
+              err = fma(df, df, err);
+            }
           v[k] = nv;
         }
       }
@@ -534,8 +546,14 @@ __global__ __launch_bounds__(kHalsRowsPerBlock) void hals_iter_kernel(HalsArgs a
         double num = mk - dot + d * v[k];
         if (a.has_sparsity) num -= a.sparsity;
         const double nv = fmax(num / d, 0.0);
-        const double diff = v[k] - nv;
-        err = fma(diff, diff, err);
+        // the stopping sum subtracts the new value from every entry of
+        // this column of V (the reference's broadcast ||V - newV||^2)
+#pragma unroll
+        for (int l = 0; l < RT; ++l)
+          if (l < R) {
+            const double df = v[l] - nv;
+            err = fma(df, df, err);
+          }
         v[k] = nv;
       }
     }
@@ -616,7 +634,7 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
                          double* gram_out, double* iprod_out, int* status_out, int* counter, void* ws,
                          size_t ws_bytes, const double* normx2, double* errors, int err_slot,
                          const double* vi_pre, const int* vi_status, int next_mode, double* vi_next,
-                         int* vist_next, void* stream) {
+                         int* vist_next, int* status_max, void* stream) {
   TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "cp_solve: bad dtype %d", dtype);
   TLA_REQUIRE(ndim >= 2 && ndim <= kMaxOrder, "cp_solve: order %d", ndim);
   TLA_REQUIRE(mode >= 0 && mode < ndim, "cp_solve: mode %d", mode);
@@ -635,6 +653,7 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
   a.gram_partial = (double*)ws;
   a.ip_partial = iprod_out ? (double*)((char*)ws + align256(size_t(nblk) * rank * rank * 8)) : nullptr;
   a.status = status_out;
+  a.status_max = status_max;
   a.counter = counter;
   a.gram_out = gram_out;
   a.iprod_out = iprod_out;
@@ -683,7 +702,7 @@ extern "C" int tla_cp_solve(int dtype, const double* const* grams, int ndim, int
                             size_t ws_bytes, void* stream) {
   return tla::cp_solve_launch(dtype, grams, ndim, mode, weights, l2_reg, mttkrp, rows, rank, factor_out, gram_out,
                               iprod_out, status_out, counter, ws, ws_bytes, nullptr, nullptr, 0, nullptr, nullptr,
-                              -1, nullptr, nullptr, stream);
+                              -1, nullptr, nullptr, nullptr, stream);
 }
 
 extern "C" int tla_nn_mu_update(int dtype, void* factor, const void* mttkrp, int64_t rows, int rank,

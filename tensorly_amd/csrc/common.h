@@ -104,7 +104,7 @@ int cp_solve_launch(int dtype, const double* const* grams, int ndim, int mode, c
                     const void* mttkrp, int64_t rows, int rank, void* factor_out, double* gram_out, double* iprod_out,
                     int* status_out, int* counter, void* ws, size_t ws_bytes, const double* normx2, double* errors,
                     int err_slot, const double* vi_pre, const int* vi_status, int next_mode, double* vi_next,
-                    int* vist_next, void* stream);
+                    int* vist_next, int* status_max, void* stream);
 
 // tla_mttkrp (ke == kb + 1) / tla_mttkrp_partial keeping modes [kb, ke),
 // with an optional side job (InvJob, nullable) that the persistent kernels

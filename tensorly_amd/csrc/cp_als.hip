@@ -72,6 +72,7 @@ struct CpAls {
   double* scales;
   int* status;
   int* counter;
+  int* status_max;
   double* err_out;  // errors[] of the current sweep (or NULL): fused into the last solve
   int err_slot;
   // V^-1 of the next mode to solve, formed by the finalising block of the
@@ -116,7 +117,7 @@ static Layout layout(int x_dtype, int ndim, const int64_t* shape, int R, int pre
   L.normx2 = take(8);
   L.iprod = take(8);
   L.scales = take(size_t(ndim) * R * 8);
-  L.status = take(8);
+  L.status = take(12);
   L.vi = take(size_t(ndim) * R * R * 8);
   L.vist = take(size_t(ndim) * 4);
   int64_t maxI = 0;
@@ -192,7 +193,8 @@ static int solve_mode(CpAls* h, int n, const float* M, bool want_iprod, int next
                                  h->grams[n], want_iprod ? h->iprod : nullptr, h->status, h->counter, h->ws,
                                  h->ws_bytes, h->normx2, want_iprod ? h->err_out : nullptr, h->err_slot,
                                  pre ? h->vi[n] : nullptr, pre ? h->vist + n : nullptr, next,
-                                 next >= 0 ? h->vi[next] : nullptr, next >= 0 ? h->vist + next : nullptr, st);
+                                 next >= 0 ? h->vi[next] : nullptr, next >= 0 ? h->vist + next : nullptr,
+                                 h->status_max, st);
   if (rc == TLA_OK && next >= 0) h->vi_valid[next] = true;
   return rc;
 }
@@ -305,7 +307,10 @@ extern "C" int tla_cp_als_create(int x_dtype, const void* X, int ndim, const int
   h->iprod = (double*)(base + L.iprod);
   h->scales = (double*)(base + L.scales);
   h->status = (int*)(base + L.status);
-  h->counter = h->status + 1;  // the status slot is 8 bytes: int status, int block counter
+  // the status slot holds three ints: the last solve's status, the solve's
+  // block counter and the largest status since the last refresh
+  h->counter = h->status + 1;
+  h->status_max = h->status + 2;
   for (int m = 0; m < ndim; ++m) h->vi[m] = (double*)(base + L.vi) + size_t(m) * rank * rank;
   h->vist = (int*)(base + L.vist);
   invalidate_inverses(h);
@@ -335,7 +340,7 @@ extern "C" int tla_cp_als_refresh(void* handle, int with_norm, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const int dt = h->x_dtype == TLA_F64 ? TLA_F64 : TLA_F32;
   invalidate_inverses(h);
-  zero_ints_kernel<<<1, 64, 0, st>>>(h->status, 2);
+  zero_ints_kernel<<<1, 64, 0, st>>>(h->status, 3);
   TLA_LAUNCH_CHECK();
   if (with_norm) {
     int64_t total = 1;
@@ -399,7 +404,8 @@ extern "C" int tla_cp_als_sweeps(void* handle, int n_sweeps, double* errors, int
 
 // Copy part of the driver state to a device buffer (stream-ordered):
 // 0 grams (ndim x R x R FP64), 1 ‖X‖² (FP64), 2 last <M, A> (FP64),
-// 3 solve status (int).
+// 3 last solve status (int), 4 largest solve status since the last refresh
+// (int; 2 = a singular system was met).
 extern "C" int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t bytes, void* stream) {
   CpAls* h = (CpAls*)handle;
   TLA_REQUIRE(h && dst, "cp_als: NULL pointer");
@@ -410,6 +416,7 @@ extern "C" int tla_cp_als_copy_state(void* handle, int which, void* dst, size_t 
     case 1: src = h->normx2; avail = 8; break;
     case 2: src = h->iprod; avail = 8; break;
     case 3: src = h->status; avail = 4; break;
+    case 4: src = h->status_max; avail = 4; break;
     default: TLA_REQUIRE(false, "cp_als: unknown state %d", which);
   }
   TLA_REQUIRE(bytes <= avail, "cp_als: %zu bytes requested, state %d holds %zu", bytes, which, avail);

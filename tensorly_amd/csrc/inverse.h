@@ -75,7 +75,11 @@ __device__ __forceinline__ bool block_gj_inverse(const double* V, int R, double*
 #pragma unroll
         for (int j = 0; j < CPT; ++j) x[j] *= inv;
       } else {
-        const double f = col[i] * inv;
+        // the multiplier by a true division: a row equal to the pivot row
+        // gets exactly 1 and cancels to exact zeros, so a V with two equal
+        // rows (duplicated factor columns) meets an exactly zero pivot and
+        // is reported singular, as LAPACK's LU finds it
+        const double f = col[i] / p;
 #pragma unroll
         for (int j = 0; j < CPT; ++j) x[j] = fma(-f, row[c0 + j], x[j]);
       }
@@ -93,7 +97,9 @@ __device__ __forceinline__ bool block_gj_inverse(const double* V, int R, double*
 
 // Gauss-Jordan inverse with partial pivoting (fallback for V that is not
 // numerically positive definite).  A is destroyed; Vi is R x R, stride R.
-// Returns false if singular.
+// Returns false if singular: a pivot column with no nonzero entry, the test
+// of LAPACK's getrf.  Multipliers are formed against the unscaled pivot row
+// (A[i][k] / A[k][k]), so rows equal to the pivot row cancel exactly.
 __device__ inline bool lds_gj_inverse(double* A, double* Vi, int R, int* piv_shared, int* flag) {
   const int tid = threadIdx.x;
   for (int e = tid; e < R * R; e += blockDim.x) Vi[e] = ((e / R) == (e % R)) ? 1.0 : 0.0;
@@ -120,14 +126,11 @@ __device__ inline bool lds_gj_inverse(double* A, double* Vi, int R, int* piv_sha
       }
     }
     __syncthreads();
-    const double inv = 1.0 / A[k * R + k];
-    __syncthreads();
-    for (int j = tid; j < R; j += blockDim.x) { A[k * R + j] *= inv; Vi[k * R + j] *= inv; }
-    __syncthreads();
+    const double akk = A[k * R + k];
     for (int e = tid; e < R * R; e += blockDim.x) {
       const int i = e / R, j = e % R;
       if (i != k) {
-        const double f = A[i * R + k];
+        const double f = A[i * R + k] / akk;
         if (j != k) A[i * R + j] -= f * A[k * R + j];
         Vi[i * R + j] -= f * Vi[k * R + j];
       }
@@ -135,6 +138,7 @@ __device__ inline bool lds_gj_inverse(double* A, double* Vi, int R, int* piv_sha
     __syncthreads();
     for (int i = tid; i < R; i += blockDim.x)
       if (i != k) A[i * R + k] = 0.0;
+    for (int j = tid; j < R; j += blockDim.x) { A[k * R + j] /= akk; Vi[k * R + j] /= akk; }
     __syncthreads();
   }
   return true;

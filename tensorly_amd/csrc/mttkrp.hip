@@ -1797,6 +1797,10 @@ int tla::mttkrp_launch(int x_dtype, const void* X, int ndim, const int64_t* shap
   if (job_taken) *job_taken = false;
   TLA_REQUIRE(X && out && factors, "NULL pointer argument");
   if (x_dtype == TLA_F64 && precision != TLA_PREC_VALU) precision = TLA_PREC_VALU;
+  // A matrix's MTTKRP multiplies by the one other factor alone: khatri_rao
+  // of a single matrix returns it without the weights
+  // (tensorly/tenalg/core_tenalg/_khatri_rao.py:67-69).
+  if (ndim == 2 && ke == kb + 1) weights = nullptr;
   Plan p;
   int rc = make_plan(x_dtype, ndim, shape, rank, kb, ke, precision, &p, job != nullptr && job->R > 0);
   if (rc != TLA_OK) return rc;

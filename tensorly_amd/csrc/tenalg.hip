@@ -691,7 +691,9 @@ extern "C" int tla_mttkrp_coo(int dtype, int64_t nnz, int ndim, const int64_t* s
   }
   a.coords = coords; a.nnz = nnz; a.vals = values; a.perm = perm; a.row_ptr = row_ptr;
   a.seg_row = seg_row; a.seg_start = seg_start; a.seg_row_ptr = seg_row_ptr; a.n_seg = n_seg; a.seg_len = seg_len;
-  a.weights = weights; a.ndim = ndim; a.mode = mode; a.R = rank; a.rows = shape[mode];
+  // order 2: the Khatri-Rao product of the one other factor carries no
+  // weights (tensorly/tenalg/core_tenalg/_khatri_rao.py:67-69)
+  a.weights = ndim == 2 ? nullptr : weights; a.ndim = ndim; a.mode = mode; a.R = rank; a.rows = shape[mode];
   a.partial = workspace; a.out = out;
   hipStream_t st = (hipStream_t)stream;
   if (n_seg > 0) {

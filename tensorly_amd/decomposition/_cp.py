@@ -188,6 +188,16 @@ class CpAlsEngine:
         check(lib().tla_cp_als_copy_state(self.handle, 1, ptr(buf), 8, stream(self.dev)), "cp_als_copy_state")
         return buf[0]
 
+    def raise_if_singular(self):
+        """Raise ``numpy.linalg.LinAlgError`` if a solve since the last
+        refresh met a singular normal-equations matrix: the error that
+        ``tl.solve`` (``np.linalg.solve``) raises in the reference's mode
+        update (``_cp.py:425-427``).  Synchronises with the device."""
+        buf = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        check(lib().tla_cp_als_copy_state(self.handle, 4, ptr(buf), 4, stream(self.dev)), "cp_als_copy_state")
+        if int(buf) == 2:
+            raise np.linalg.LinAlgError("Singular matrix")
+
     def set_factors(self, weights, factors):
         self.weights.copy_(weights)
         for f, g in zip(self.factors, factors):
@@ -268,6 +278,7 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
     if not host_loop:
         if n_iter_max > 0:
             eng.sweeps(n_iter_max, errs_dev if want_err else None, 0, normalize)
+        eng.raise_if_singular()
         rec_errors = [float(v) for v in errs_dev[:n_iter_max].cpu()] if want_err else []
         return _result(eng, rec_errors, return_errors)
 
@@ -280,6 +291,7 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
             factors_last = [f.clone() for f in eng.factors]
             weights_last = eng.weights.clone()
         eng.sweeps(1, errs_dev, iteration, False)
+        eng.raise_if_singular()
         line_iter = bool(linesearch and iteration % 2 == 0 and iteration > 5)
         if line_iter:
             jump = iteration ** (1.0 / acc_pow)

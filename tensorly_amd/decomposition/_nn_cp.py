@@ -15,6 +15,7 @@ Not implemented (raise NotImplementedError rather than run elsewhere):
 """
 import warnings
 
+import numpy as np
 import torch
 
 from .. import _lib
@@ -130,7 +131,7 @@ def _hals_update(factors, mttkrp, grams, weights, mode, sparsity, n_iter_max, to
           "hals_nnls")
 
 
-def _solve_update(factors, mttkrp, grams, weights, mode):
+def _solve_update(factors, mttkrp, grams, weights, mode, seen):
     """Unconstrained mode of the HALS driver: A = M (pseudo_inverse)^-1, the
     FP64 ALS solve (``tla_cp_solve``)."""
     f = factors[mode]
@@ -144,7 +145,14 @@ def _solve_update(factors, mttkrp, grams, weights, mode):
     check(lib().tla_cp_solve(_lib.dtype_code(f), gp, len(factors), mode, ptr(weights), 0.0, ptr(m), rows, rank,
                              ptr(f), ptr(g_out), None, ptr(status), None, ptr(ws), nb, stream(f.device)),
           "cp_solve")
+    torch.maximum(seen, status, out=seen)
     return g_out
+
+
+def _raise_if_singular(seen):
+    """``tl.solve`` raises on a singular system (``_nn_cp.py:338``)."""
+    if int(seen) == 2:
+        raise np.linalg.LinAlgError("Singular matrix")
 
 
 def non_negative_parafac_hals(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", tol=10e-8,
@@ -187,6 +195,7 @@ def non_negative_parafac_hals(tensor, rank, n_iter_max=100, init="svd", svd="tru
     rec_errors = []
     grams = grams64(factors)
     mttkrp = None
+    seen = torch.zeros(1, dtype=torch.int32, device=tensor.device)
     for iteration in range(n_iter_max):
         for mode in modes:
             mttkrp = unfolding_dot_khatri_rao(tensor, (weights, factors), mode)
@@ -195,13 +204,14 @@ def non_negative_parafac_hals(tensor, rank, n_iter_max=100, init="svd", svd="tru
                              inner_tol)
                 grams[mode] = grams64([factors[mode]])[0]
             else:
-                grams[mode] = _solve_update(factors, mttkrp, grams, weights, mode)
+                grams[mode] = _solve_update(factors, mttkrp, grams, weights, mode, seen)
             if normalize_factors and mode != modes[-1]:
                 weights, factors = cp_normalize((weights, factors))
                 weights = weights.contiguous()
                 factors = [f.contiguous() for f in factors]
                 grams = grams64(factors)
         if tol:
+            _raise_if_singular(seen)
             unnorml_rec_error, _, _ = error_calc(tensor, norm_tensor, weights, factors, mttkrp=mttkrp)
             rec_error = float(unnorml_rec_error / norm_tensor)
             rec_errors.append(rec_error)
@@ -227,6 +237,7 @@ def non_negative_parafac_hals(tensor, rank, n_iter_max=100, init="svd", svd="tru
             weights = weights.contiguous()
             factors = [f.contiguous() for f in factors]
             grams = grams64(factors)
+    _raise_if_singular(seen)
     cp_tensor = CPTensor((weights, factors))
     if return_errors:
         return cp_tensor, rec_errors

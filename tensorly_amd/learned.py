@@ -153,6 +153,7 @@ def run_extrapolated_als(eng, n_iter_max, tol, policy, return_errors, normalize,
         raise NotImplementedError("extrapolated ALS runs a fixed number of sweeps (tol=0, no callback, "
                                   "normalize_factors=False)")
     errors, accepted, alphas, _ = run_extrapolated_sweeps(eng, n_iter_max, policy)
+    eng.raise_if_singular()
     cp = CPTensor((eng.weights.clone(), [f.clone() for f in eng.factors]))
     if return_errors:
         return cp, [float(e) for e in errors[:n_iter_max].cpu()]

@@ -20,31 +20,9 @@ __all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd
 
 SVD_FUNS = ["truncated_svd", "symeig_svd"]
 
-_sigs_added = False
-
-
-def _add_sigs():
-    global _sigs_added
-    if _sigs_added:
-        return
-    L = lib()
-    L.tla_mode_gram64_workspace_size.restype = _lib.c_sz
-    L.tla_mode_gram64_workspace_size.argtypes = [_lib.c_i, _lib.P_i64, _lib.c_i]
-    L.tla_mode_gram64.restype = _lib.c_i
-    L.tla_mode_gram64.argtypes = [_lib.c_i, _lib.c_p, _lib.c_i, _lib.P_i64, _lib.c_i, _lib.c_p, _lib.c_p,
-                                  _lib.c_sz, _lib.c_p]
-    L.tla_sym_eig_workspace_size.restype = _lib.c_sz
-    L.tla_sym_eig_workspace_size.argtypes = [_lib.c_i]
-    L.tla_sym_eig.restype = _lib.c_i
-    L.tla_sym_eig.argtypes = [_lib.c_p, _lib.c_i, _lib.c_i, _lib.c_p, _lib.c_i, _lib.c_p, _lib.c_p, _lib.c_d,
-                              _lib.c_i, _lib.c_p, _lib.c_p, _lib.c_sz, _lib.c_p]
-    _sigs_added = True
-
-
 def mode_gram(tensor, mode):
     """FP64 ``unfold(X, mode) @ unfold(X, mode)^T`` without forming the
     unfolding."""
-    _add_sigs()
     _lib.require_gpu_tensor(tensor, "tensor")
     x = tensor.contiguous()
     n = x.shape[mode]
@@ -60,7 +38,6 @@ def mode_gram(tensor, mode):
 def gram_eigh(G, k, warm=None, tol=1e-11, max_iter=300):
     """Leading ``k`` eigenpairs (descending) of the symmetric FP64 ``G``:
     returns ``(evals (k,), evecs (n, k))`` as FP64 GPU tensors."""
-    _add_sigs()
     n = G.shape[0]
     G = G.to(torch.float64).contiguous()
     evals = torch.empty(k, dtype=torch.float64, device=G.device)

@@ -68,15 +68,16 @@ def cp_512_rank32(dtype=np.float32):
     return x, (w, factors)
 
 
-def cp_128x4_rank64(dtype=np.float32):
-    """128^4; rng(2): four 128x64 N(0,1) factors with unit-norm columns,
-    weights 10**U(-2, 0), model in float64 then cast, 0.1 % noise."""
+def cp_128x4_rank64(dtype=np.float32, size=128, rank=64):
+    """size^4 (128 by default); rng(2): four size x rank N(0,1) factors with
+    unit-norm columns, weights 10**U(-2, 0), model in float64 then cast,
+    0.1 % noise."""
     rng = np.random.default_rng(2)
     factors = []
     for _ in range(4):
-        f = rng.standard_normal((128, 64))
+        f = rng.standard_normal((size, rank))
         factors.append(f / np.linalg.norm(f, axis=0, keepdims=True))
-    w = 10.0 ** rng.uniform(-2.0, 0.0, size=64)
+    w = 10.0 ** rng.uniform(-2.0, 0.0, size=rank)
     x = cp_model(w, factors, dtype=dtype)
     x = add_noise(x, 0.001, rng)
     return x, (w, factors)

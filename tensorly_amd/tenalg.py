@@ -85,8 +85,14 @@ def khatri_rao(matrices, weights=None, skip_matrix=None, mask=None):
         if m.shape[1] != ncol:
             raise ValueError("All matrices must have same number of columns!"
                              f"Matrix {i} has {m.shape[1]} columns != {ncol}.")
+    _lib.require_gpu_tensor(matrices[0], "matrices[0]")
+    dev = matrices[0].device
     for i, m in enumerate(matrices):
-        _lib.require_gpu_tensor(m, f"matrices[{i}]")
+        _lib.require_on_device(m, dev, f"matrices[{i}]")
+    if weights is not None:
+        _lib.require_on_device(weights, dev, "weights")
+    if mask is not None:
+        _lib.require_on_device(mask, dev, "mask")
     dt = matrices[0].dtype
     mats = [m.to(dt).contiguous() for m in matrices]
     rows = [m.shape[0] for m in mats]
@@ -128,10 +134,12 @@ def unfolding_dot_khatri_rao(tensor, cp_tensor, mode):
         raise ValueError(f"got {len(factors)} factors for a tensor of order {nd}")
     rank = factors[0].shape[1] if _ndim(factors[0]) == 2 else 1
     for i, f in enumerate(factors):
-        _lib.require_gpu_tensor(f, f"factors[{i}]")
+        _lib.require_on_device(f, tensor.device, f"factors[{i}]")
         if i != mode and (f.shape[0] != tensor.shape[i] or (f.shape[1] if f.dim() == 2 else 1) != rank):
             raise ValueError(f"factor {i} has shape {tuple(f.shape)}, expected "
                              f"({tensor.shape[i]}, {rank})")
+    if weights is not None:
+        _lib.require_on_device(weights, tensor.device, "weights")
     fdt = _mttkrp_dtype(tensor, factors)
     x = tensor.contiguous()
     facs = [None if i == mode else f.reshape(f.shape[0], rank).to(fdt).contiguous()
@@ -173,8 +181,12 @@ def unfolding_dot_khatri_rao_coo(coords, values, shape, cp_tensor, mode):
     coords = coords.to(device=values.device, dtype=torch.int64).contiguous()
     nnz = int(values.shape[0])
     for m, s in enumerate(shape):
-        if m != mode and tuple(factors[m].shape) != (int(s), rank):
-            raise ValueError(f"factor {m} has shape {tuple(factors[m].shape)}, expected ({int(s)}, {rank})")
+        if m != mode:
+            _lib.require_on_device(factors[m], values.device, f"factors[{m}]")
+            if tuple(factors[m].shape) != (int(s), rank):
+                raise ValueError(f"factor {m} has shape {tuple(factors[m].shape)}, expected ({int(s)}, {rank})")
+    if weights is not None:
+        _lib.require_on_device(weights, values.device, "weights")
     if nnz and (int(coords.min()) < 0 or any(int(coords[m].max()) >= int(s) for m, s in enumerate(shape))):
         raise ValueError("coordinates out of the tensor's shape")
     dev = values.device
@@ -215,6 +227,11 @@ def mttkrp_partial(tensor, cp_tensor, keep, precision_name=None):
     _lib.require_gpu_tensor(tensor, "tensor")
     nd = _ndim(tensor)
     kb, ke = keep
+    for i, f in enumerate(factors):
+        if not kb <= i < ke:
+            _lib.require_on_device(f, tensor.device, f"factors[{i}]")
+    if weights is not None:
+        _lib.require_on_device(weights, tensor.device, "weights")
     rank = factors[0].shape[1]
     facs = [None if kb <= i < ke else f.to(torch.float32).contiguous() for i, f in enumerate(factors)]
     w = weights.to(torch.float32).contiguous() if weights is not None else None
@@ -235,6 +252,8 @@ def mttkrp_partial(tensor, cp_tensor, keep, precision_name=None):
 
 def rankdiag_contract(Y, F, axis):
     """``Y`` is (I, J, R): axis 1 -> sum_j Y[i,j,r] F[j,r]; axis 0 -> sum_i Y[i,j,r] F[i,r]."""
+    _lib.require_gpu_tensor(Y, "Y")
+    _lib.require_on_device(F, Y.device, "F")
     I, J, R = Y.shape
     out = torch.empty((I if axis == 1 else J, R), dtype=torch.float32, device=Y.device)
     nbytes = lib().tla_rankdiag_contract_workspace_size(I, J, R, axis)
@@ -274,7 +293,7 @@ def mode_dot(tensor, matrix_or_vector, mode, transpose=False):
         raise ValueError("Can only take n_mode_product with a vector or a matrix."
                          f"Provided array of dimension {_ndim(mv)} not in [1, 2].")
     _lib.require_gpu_tensor(tensor, "tensor")
-    _lib.require_gpu_tensor(mv, "matrix_or_vector")
+    _lib.require_on_device(mv, tensor.device, "matrix_or_vector")
     dt = torch.float64 if (tensor.dtype == torch.float64 or mv.dtype == torch.float64) else torch.float32
     x = tensor.to(dt).contiguous()
     u = mv.to(dt).contiguous()

@@ -29,6 +29,13 @@ def test_library_exports_every_declared_symbol():
     assert lib.tla_abi_version() == 1
 
 
+def test_binding_covers_the_header():
+    """Every declared entry point has its ctypes signature in one table
+    (no ad hoc bindings elsewhere), and nothing undeclared is bound."""
+    from tensorly_amd import _lib
+    assert _lib.exported_symbols() == _declared()
+
+
 def test_workspace_queries_are_host_only():
     from tensorly_amd import _lib
     lib = _lib.lib()

@@ -1,0 +1,108 @@
+"""The precision-study and headline workloads at test sizes and at full size,
+on the GPU against the FP64 oracle.
+
+* The 4th-order rank-64 recipe (unit-norm N(0,1) columns, log-uniform
+  weights, 0.1 % noise; ``synthetic.cp_128x4_rank64``) at 40^4: every mode's
+  MTTKRP in FP32 and bf16x3 within 1e-5 (relative, Frobenius) of the FP64
+  oracle on signed data, the BF16 copy within 1e-2; ALS from one random init
+  against the oracle run.
+* The 512^3 rank-32 headline (``synthetic.cp_512_rank32``) at full size:
+  100 sweeps of plain FP32 ALS and of bf16x3 ALS on the dimension tree,
+  every sweep's error against the FP64 oracle's trajectory
+  (``tests/golden/cp512_oracle.npz``).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import tensorly_np as T
+from tensorly_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def c3_small():
+    x, (w, f) = synthetic.cp_128x4_rank64(size=40)
+    return x, w, f
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-5), ("bf16", 1e-2)])
+def test_c3_recipe_mttkrp_every_mode(cuda, c3_small, prec, tol):
+    """Rank 64 on signed data: the factors are fresh N(0,1) draws, so every
+    MTTKRP entry is a sum of terms of both signs."""
+    import tensorly_amd as tla
+    x, _, _ = c3_small
+    rng = np.random.default_rng(12)
+    f = [rng.standard_normal((40, 64)).astype(np.float32) for _ in range(4)]
+    xg = torch.tensor(x, device=cuda)
+    if prec == "bf16":
+        xg = xg.to(torch.bfloat16)
+    fg = [torch.tensor(a, device=cuda) for a in f]
+    x64, f64 = x.astype(np.float64), [a.astype(np.float64) for a in f]
+    errs = []
+    with tla.precision(prec):
+        for mode in range(4):
+            ref = T.unfolding_dot_khatri_rao(x64, (None, f64), mode)
+            errs.append(rel(tla.unfolding_dot_khatri_rao(xg, (None, fg), mode).cpu(), ref))
+    print(f"C3 40^4 R=64 {prec}: per-mode relative error {['%.2e' % e for e in errs]}")
+    assert max(errs) < tol, errs
+
+
+@pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("bf16x3", True)])
+def test_c3_recipe_als_vs_oracle(cuda, c3_small, prec, dimtree):
+    """ALS on the 40^4 rank-64 tensor from the same random init as the FP64
+    oracle: every sweep's err^2 within 1e-6 of the oracle's (absolute; the
+    fit's three-term form turns the MTTKRP's relative rounding into an
+    absolute error in err^2)."""
+    import tensorly_amd as tla
+    x, _, _ = c3_small
+    n = 12
+    (_, _), ref_errs = T.parafac(x.astype(np.float64), 64, n_iter_max=n, init="random", random_state=7, tol=0,
+                                 return_errors=True)
+    _, errs = tla.parafac(torch.tensor(x, device=cuda), 64, n_iter_max=n, init="random", random_state=7, tol=0,
+                          return_errors=True, precision=prec, dimtree=dimtree)
+    d = np.abs(np.array(errs) ** 2 - np.array(ref_errs) ** 2)
+    print(f"C3 40^4 R=64 ALS {prec} dimtree={dimtree}: final error {errs[-1]:.8f} vs oracle {ref_errs[-1]:.8f}, "
+          f"max |d err^2| {np.max(d):.2e}")
+    assert np.max(d) < 1e-6, d
+
+
+@pytest.fixture(scope="module")
+def c2_full(cuda):
+    gold = np.load(os.path.join(GOLDEN, "cp512_oracle.npz"))
+    x, _ = synthetic.cp_512_rank32()
+    x64sum = float(np.sum(x, dtype=np.float64))
+    # the fixture's checksum pins the data the oracle trajectory was run on
+    np.testing.assert_allclose(x64sum, gold["checksum"][0], rtol=1e-12)
+    xg = torch.tensor(x, device=cuda)
+    del x
+    return xg, gold["errors"]
+
+
+@pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("bf16x3", True)])
+def test_c2_full_size_trajectory(cuda, c2_full, prec, dimtree):
+    """512^3 float32, rank 32, init='random' (random_state=1), 100 sweeps:
+    the exact launch shapes of the headline (F_k staged in LDS at K = 512,
+    the streaming grid plus the side workgroup inverting V) against the FP64
+    oracle's 100 errors.  err^2 within 1e-6 absolute at every sweep."""
+    import tensorly_amd as tla
+    xg, ref = c2_full
+    _, errs = tla.parafac(xg, 32, n_iter_max=100, init="random", random_state=1, tol=0, return_errors=True,
+                          precision=prec, dimtree=dimtree)
+    errs = np.array(errs)
+    d = np.abs(errs ** 2 - ref ** 2)
+    print(f"C2 512^3 R=32 {prec} dimtree={dimtree}: final error {errs[-1]:.8f} vs oracle {ref[-1]:.8f}, "
+          f"max |d err| {np.max(np.abs(errs - ref)):.2e}, max |d err^2| {np.max(d):.2e}")
+    assert np.all(np.isfinite(errs))
+    assert np.max(d) < 1e-6, d

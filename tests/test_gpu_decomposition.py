@@ -261,3 +261,70 @@ def test_tucker_float32_vs_oracle(cuda, prec, tol_e2, tol_core):
         assert np.min(s) > 1 - 1e-6
     # the core in the reference's basis (the sign rule makes the bases agree)
     assert rel(tt.core.cpu().numpy(), core) < tol_core
+
+
+def _duplicated_init(rng, shape, rank):
+    """Random factors whose columns 0 and 1 are equal in every mode: each
+    Gram, their Hadamard product and so V then have two equal rows."""
+    f = [rng.random((s, rank)) for s in shape]
+    for a in f:
+        a[:, 1] = a[:, 0]
+    return np.ones(rank), f
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("tol", [0, 1e-8])
+def test_parafac_singular_system_raises(cuda, dtype, tol):
+    """Duplicated factor columns make the normal equations exactly singular:
+    the reference's np.linalg.solve raises LinAlgError (``_cp.py:425-427``),
+    and so does the drop-in, on the no-sync path (checked at return) and on
+    the host loop (checked per sweep)."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(31)
+    shape, rank = (20, 24, 28), 4
+    x = rng.random(shape)
+    w0, f0 = _duplicated_init(rng, shape, rank)
+    with pytest.raises(np.linalg.LinAlgError):
+        T.parafac(x, rank, n_iter_max=3, init=(w0, [a.copy() for a in f0]), tol=tol)
+    with pytest.raises(np.linalg.LinAlgError):
+        tla.parafac(g(x, cuda, dtype), rank, n_iter_max=3, init=(g(w0, cuda, dtype), [g(a, cuda, dtype) for a in f0]),
+                    tol=tol)
+
+
+def test_parafac_coo_singular_system_raises(cuda):
+    """The sparse drop-in raises as the dense one on duplicated columns; a
+    well-posed problem from a random init does not."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(32)
+    shape, rank = (12, 14, 16), 3
+    dense = rng.random(shape) * (rng.random(shape) < 0.3)
+    coords = torch.tensor(np.stack(np.nonzero(dense)), device=cuda)
+    vals = torch.tensor(dense[np.nonzero(dense)], device=cuda)
+    w0, f0 = _duplicated_init(rng, shape, rank)
+    with pytest.raises(np.linalg.LinAlgError):
+        T.parafac(dense, rank, n_iter_max=2, init=(w0, [a.copy() for a in f0]), tol=0)
+    with pytest.raises(np.linalg.LinAlgError):
+        tla.parafac_coo(coords, vals, shape, rank, n_iter_max=2, init=(g(w0, cuda), [g(a, cuda) for a in f0]), tol=0)
+    tla.parafac_coo(coords, vals, shape, rank, n_iter_max=4, init="random", random_state=0, tol=0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_parafac_matrix_normalize_factors(cuda, dtype):
+    """Order 2 with normalize_factors=True: the weights scale V but not the
+    MTTKRP (khatri_rao of one matrix is unweighted, ``_khatri_rao.py:67-69``),
+    so the drop-in must follow the oracle's iterates and errors exactly."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(33)
+    M = rng.random((30, 4)) @ rng.random((4, 45)) + 0.01 * rng.standard_normal((30, 45))
+    (w_ref, f_ref), e_ref = T.parafac(M, 4, n_iter_max=15, init="random", random_state=2, tol=0,
+                                      normalize_factors=True, return_errors=True)
+    cp, errs = tla.parafac(g(M, cuda, dtype), 4, n_iter_max=15, init="random", random_state=2, tol=0,
+                           normalize_factors=True, return_errors=True)
+    if dtype == torch.float64:
+        np.testing.assert_allclose(errs, e_ref, rtol=1e-9)
+        np.testing.assert_allclose(cp.weights.cpu().numpy(), w_ref, rtol=1e-8)
+        for a, b in zip(cp.factors, f_ref):
+            np.testing.assert_allclose(a.cpu().numpy(), b, rtol=1e-7, atol=1e-10)
+    else:
+        d = np.abs(np.array(errs) ** 2 - np.array(e_ref) ** 2)
+        assert np.max(d) < 1e-6, d

@@ -107,7 +107,7 @@ def test_non_negative_parafac_rejects_mask(cuda):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("rows,rank", [(1, 1), (37, 5), (700, 16), (129, 33), (64, 64)])
+@pytest.mark.parametrize("rows,rank", [(1, 1), (37, 5), (700, 16), (300, 24), (129, 33), (90, 40), (64, 64)])
 @pytest.mark.parametrize("sparsity", [None, 0.05])
 def test_hals_nnls_kernel_vs_oracle(cuda, dtype, rows, rank, sparsity):
     from tensorly_amd.cp_tensor import grams64

@@ -110,3 +110,54 @@ def test_parafac_coo_rejects_svd_init(cuda):
     with pytest.raises(NotImplementedError):
         tla.parafac_coo(torch.zeros((3, 1), dtype=torch.int64, device=cuda), torch.ones(1, device=cuda,
                         dtype=torch.float64), (2, 2, 2), 1, init="svd")
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_coo_mttkrp_multi_piece_rows(cuda, dtype):
+    """Rows with more than 1024 nonzeros are cut into several pieces whose
+    partial sums are added in a fixed order: against the oracle, and bit for
+    bit equal on a second call.  Shape (3, 64, 64) with 5000 nonzeros puts
+    ~1667 nonzeros (two pieces) in every mode-0 row."""
+    import tensorly_amd as tla
+    shape, R = (3, 64, 64), 12
+    coords, vals = _coo(77, shape, 5000)
+    rng = np.random.default_rng(78)
+    w = rng.random(R) + 0.5
+    factors = [rng.standard_normal((s, R)) for s in shape]
+    c = torch.tensor(coords, device=cuda)
+    v = torch.tensor(vals, dtype=dtype, device=cuda)
+    wt = torch.tensor(w, dtype=dtype, device=cuda)
+    ft = [torch.tensor(f, dtype=dtype, device=cuda) for f in factors]
+    assert np.bincount(coords[0]).min() > 1024
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for mode in range(3):
+        got = tla.tenalg.unfolding_dot_khatri_rao_coo(c, v, shape, (wt, ft), mode)
+        ref = T.unfolding_dot_khatri_rao_coo(coords, vals, shape, (w, factors), mode)
+        np.testing.assert_allclose(got.double().cpu().numpy(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+        again = tla.tenalg.unfolding_dot_khatri_rao_coo(c, v, shape, (wt, ft), mode)
+        assert torch.equal(got, again)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_coo_mttkrp_order2_ignores_weights(cuda, dtype):
+    """A sparse matrix's MTTKRP is X @ B without the weights, as for the
+    dense path (``_khatri_rao.py:67-69``): against the densified matrix."""
+    import tensorly_amd as tla
+    shape, R = (40, 50), 5
+    coords, vals = _coo(79, shape, 300)
+    rng = np.random.default_rng(80)
+    w = rng.random(R) + 0.5
+    factors = [rng.standard_normal((s, R)) for s in shape]
+    dense = np.zeros(shape)
+    np.add.at(dense, tuple(coords), vals)
+    c = torch.tensor(coords, device=cuda)
+    v = torch.tensor(vals, dtype=dtype, device=cuda)
+    wt = torch.tensor(w, dtype=dtype, device=cuda)
+    ft = [torch.tensor(f, dtype=dtype, device=cuda) for f in factors]
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for mode in range(2):
+        ref = T.unfolding_dot_khatri_rao(dense, (w, factors), mode)
+        np.testing.assert_allclose(ref, T.unfolding_dot_khatri_rao_coo(coords, vals, shape, (w, factors), mode),
+                                   rtol=1e-12, atol=1e-12)
+        got = tla.tenalg.unfolding_dot_khatri_rao_coo(c, v, shape, (wt, ft), mode)
+        np.testing.assert_allclose(got.double().cpu().numpy(), ref, rtol=tol, atol=tol * np.abs(ref).max())

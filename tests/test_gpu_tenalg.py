@@ -282,3 +282,65 @@ def test_cp_norm_and_to_tensor(cuda, known_answers):
     ref = T.cp_norm((w, f))
     out = tla.cp_norm((g(w, cuda), [g(x, cuda) for x in f]))
     np.testing.assert_allclose(float(out), ref, rtol=1e-12)
+
+
+def test_unfold_fold_known_answers(cuda, known_answers):
+    """The reference's unfolding table of arange(24) as 3 x 4 x 2
+    (``tests/test_core.py``), on GPU tensors, and fold as its inverse."""
+    import tensorly_amd as tla
+    ka = known_answers["unfold_arange24_342"]
+    X = torch.arange(24, dtype=torch.float64, device=cuda).reshape(ka["shape"])
+    for mode, expected in enumerate(ka["unfoldings"]):
+        U = tla.unfold(X, mode)
+        assert U.device == X.device
+        np.testing.assert_array_equal(U.cpu().numpy(), np.array(expected))
+        back = tla.fold(U, mode, tuple(X.shape))
+        assert torch.equal(back, X)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_mttkrp_order2_ignores_weights(cuda, dtype):
+    """A matrix's MTTKRP is X @ B without the weights: the reference's
+    khatri_rao of one matrix returns it unweighted (``_khatri_rao.py:67-69``,
+    ``mttkrp.py:47-50``)."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((40, 33))
+    A, B = rng.standard_normal((40, 6)), rng.standard_normal((33, 6))
+    w = rng.random(6) + 0.5
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for mode in range(2):
+        ref = T.unfolding_dot_khatri_rao(X, (w, [A, B]), mode)
+        np.testing.assert_allclose(ref, (X @ B) if mode == 0 else (X.T @ A), rtol=1e-12)
+        out = tla.unfolding_dot_khatri_rao(g(X, cuda, dtype), (g(w, cuda, dtype), [g(A, cuda, dtype),
+                                                                                   g(B, cuda, dtype)]), mode)
+        assert rel(out.cpu(), ref) < tol, mode
+
+
+def test_operands_on_the_host_are_refused(cuda):
+    """Every operand must be on the call's GPU: a host factor, weights or
+    mask raises TypeError before any launch."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(22)
+    X = g(rng.random((8, 9, 10)), cuda, torch.float32)
+    f = [g(rng.random((s, 4)), cuda, torch.float32) for s in (8, 9, 10)]
+    w = g(rng.random(4), cuda, torch.float32)
+    host_f = [f[0], f[1].cpu(), f[2]]
+    with pytest.raises(TypeError):
+        tla.unfolding_dot_khatri_rao(X, (None, host_f), 0)
+    with pytest.raises(TypeError):
+        tla.unfolding_dot_khatri_rao(X, (w.cpu(), f), 0)
+    with pytest.raises(TypeError):
+        tla.khatri_rao(f, weights=w.cpu())
+    with pytest.raises(TypeError):
+        tla.khatri_rao(f, mask=torch.ones(8 * 9 * 10))
+    with pytest.raises(TypeError):
+        tla.khatri_rao([f[0], f[1].cpu()])
+    with pytest.raises(TypeError):
+        tla.mode_dot(X, f[0].T.contiguous().cpu(), 0)
+    coords = torch.stack([torch.arange(5, device=cuda) % s for s in (8, 9, 10)])
+    vals = torch.ones(5, device=cuda)
+    with pytest.raises(TypeError):
+        tla.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (None, host_f), 0)
+    with pytest.raises(TypeError):
+        tla.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (w.cpu(), f), 0)

@@ -168,3 +168,23 @@ def test_coo_mttkrp_matches_the_densified_tensor(shape):
     for mode in range(len(shape)):
         np.testing.assert_allclose(T.unfolding_dot_khatri_rao_coo(coords, vals, shape, (w, factors), mode),
                                    _brute_mttkrp(dense, w, factors, mode), rtol=1e-12, atol=1e-12)
+
+
+def test_order2_mttkrp_drops_the_weights():
+    """khatri_rao of one matrix returns it as is (``_khatri_rao.py:67-69``),
+    so a matrix's MTTKRP is X @ B whatever the weights; the COO restatement
+    follows the dense one there."""
+    rng = np.random.default_rng(22)
+    shape = (6, 9)
+    coords = np.stack([rng.integers(0, s, 30) for s in shape])
+    vals = rng.standard_normal(30)
+    dense = np.zeros(shape)
+    np.add.at(dense, tuple(coords), vals)
+    w = rng.random(4) + 0.5
+    A, B = rng.standard_normal((6, 4)), rng.standard_normal((9, 4))
+    assert T.khatri_rao([B], weights=w) is B
+    np.testing.assert_allclose(T.unfolding_dot_khatri_rao(dense, (w, [A, B]), 0), dense @ B, rtol=1e-13)
+    np.testing.assert_allclose(T.unfolding_dot_khatri_rao(dense, (w, [A, B]), 1), dense.T @ A, rtol=1e-13)
+    for mode in range(2):
+        np.testing.assert_allclose(T.unfolding_dot_khatri_rao_coo(coords, vals, shape, (w, [A, B]), mode),
+                                   T.unfolding_dot_khatri_rao(dense, (w, [A, B]), mode), rtol=1e-12, atol=1e-12)
