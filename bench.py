@@ -1,29 +1,34 @@
 """CP-ALS benchmark on one MI355X: 512^3 float32 tensor, rank 32.
 
 A step is one CP-ALS sweep (every mode's fused MTTKRP, FP64 Gram-Hadamard
-Cholesky solve and the FP64 fit) over the resident 512^3 tensor.  The timed
+Gauss-Jordan solve and the FP64 fit) over the resident 512^3 tensor.  The timed
 configuration is the fast path (bf16x3 MTTKRP on MFMA with the dimension-tree
 sweep).  ``value`` is MTTKRP-equivalent throughput of the timed sweeps:
 3 MTTKRPs x 537,067,520 algorithmic bytes (tensor read once, factors and
 output) per sweep / wall time.
 
 Also reported on the same JSON line:
-  * time_to_fit: wall time until the relative error is within 1e-4 of the FP64
-    oracle's error after 100 sweeps (tests/golden/cp512_oracle.npz), for plain
-    FP32 sweeps, plain bf16x3 dimension-tree sweeps and bf16x3 dimension-tree
-    sweeps with the learned extrapolation (trained network in
-    tensorly_amd/models/extrapolation_mlp.npz); speedup_vs_fp32 is
+  * sweep: the bytes a dimension-tree sweep actually moves and their rate;
+  * time_to_fit: wall time until the relative error is within 1e-4 of the
+    converged FP64 oracle fit (tests/golden/cp512_converged.npz: the oracle
+    run until its error changed by < 1e-9), for plain FP32 sweeps, plain
+    bf16x3 dimension-tree sweeps, bf16x3 dimension-tree sweeps with a fixed
+    extrapolation step alpha = 4 and with the learned extrapolation (trained
+    network in tensorly_amd/models/extrapolation_mlp.npz); speedup_vs_fp32 is
     fp32_plain / bf16x3+learned;
   * batched: the 4096-problem batched workload (sizes U{8..64}, ranks
     U{2..16}, congruence 0.9, 1 % noise, seed 4) run to |delta rel. error| <
     1e-7 in one persistent launch per policy (plain ALS, the reference line
-    search, learned extrapolation): wall time, sweeps, tensor passes and the
-    fallback (rejected-step) rate;
+    search, fixed step alpha = 4, learned extrapolation): wall time, sweeps,
+    tensor passes and the fallback (rejected-step) rate, and the numpy
+    restatement's plain ALS and line search timed problem by problem on a
+    sample;
   * roofline: the dominant kernel's algorithmic bytes / average duration
     (HIP events on the launch stream) against 8 TB/s;
   * cpu_baseline: the numpy restatement of the reference path (unfold copy,
     materialised Khatri-Rao, np.dot, Hadamard of Grams, solve) timed on the
-    host in float32, per full sweep and per MTTKRP of each mode.
+    host in float32 from the same init until it reaches the time-to-fit
+    target: per sweep, time to fit, and per MTTKRP of each mode.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -43,6 +48,12 @@ METRIC = "CP-ALS time-to-fit 512^3 R=32: bf16x3+learned vs plain FP32; MTTKRP GB
 SHAPE = (512, 512, 512)
 RANK = 32
 MTTKRP_BYTES = 512 ** 3 * 4 + (512 + 512) * RANK * 4 + 512 * RANK * 4  # 537,067,520
+# the bytes a dimension-tree sweep moves: the partial contraction of modes 0-1
+# (tensor + C in, T01 = 512 x 512 x R out), the two rank-diagonal contractions
+# of T01 (T01 + one factor in, one MTTKRP out each) and the last-mode MTTKRP
+T01_BYTES = 512 ** 3 * 4 + 512 * RANK * 4 + 512 * 512 * RANK * 4  # 570,490,880
+RANKDIAG_BYTES = 512 * 512 * RANK * 4 + 2 * 512 * RANK * 4  # 33,685,504
+SWEEP_BYTES_DIMTREE = T01_BYTES + 2 * RANKDIAG_BYTES + MTTKRP_BYTES  # 1,174,929,408
 HBM_PEAK_GBS = 8000.0
 
 
@@ -53,7 +64,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--quick", action="store_true", help="skip time-to-fit and the CPU baseline")
-    p.add_argument("--fit-sweeps", type=int, default=100)
+    p.add_argument("--fit-sweeps", type=int, default=300)
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     p.add_argument("--kernel-only", choices=["partial01", "last", "mode0", "copy"],
                    help="only launch one kernel --iters times on the 512^3 data (for counter runs)")
@@ -253,6 +264,27 @@ def precision_study(torch, dev):
                          "gbs": round((xbytes + 4 * 128 * R * 4) / ms / 1e6, 1)}
         out["mttkrp_time"][prec] = row
         log(f"[study] MTTKRP time {prec}: {row}")
+    # CPU baseline: the numpy restatement's float32 MTTKRP (unfold copy,
+    # materialised 2,097,152 x 64 Khatri-Rao product, np.dot) per mode, median
+    # of 3 after a warm-up, on all host threads
+    from threadpoolctl import threadpool_limits
+    cores = os.cpu_count() or 1
+    f32 = [f.astype(np.float32) for f in init]
+    cpu_row = {}
+    with threadpool_limits(cores):
+        for mode in range(4):
+            T.unfolding_dot_khatri_rao(x, (None, f32), mode)
+            tm = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                T.unfolding_dot_khatri_rao(x, (None, f32), mode)
+                tm.append(time.perf_counter() - t1)
+            cpu_row[f"mode{mode}"] = {"ms": round(float(np.median(tm)) * 1e3, 2),
+                                      "tflops": round(gflop / (float(np.median(tm)) * 1e3), 4)}
+    out["cpu_baseline"] = {"kind": "port", "cores": cores, "mttkrp_fp32": cpu_row,
+                           "sample": "numpy restatement's float32 MTTKRP per mode on the 128^4 tensor, median of 3 "
+                                     "after a warm-up"}
+    log(f"[study] CPU MTTKRP: {cpu_row}")
     ones = torch.ones(R, dtype=torch.float32, device=dev)
 
     def als(prec, f0, n, x=X, w=ones, normalize=False):
@@ -349,13 +381,17 @@ def load_model(torch, dev, path=None):
     return ExtrapolationMLP(np.load(path)["params"], device=dev)
 
 
-def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch):
-    """bf16x3 dimension-tree sweeps with the learned step: find the first
-    sweep slot whose error reaches the target, then time a fresh run of
-    exactly that many slots (the run is deterministic)."""
+def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch, alpha_fixed=None):
+    """bf16x3 dimension-tree sweeps with the learned step (or, with
+    ``alpha_fixed``, a fixed step and no network): find the first sweep slot
+    whose error reaches the target, then time a fresh run of exactly that
+    many slots (the run is deterministic)."""
     from tensorly_amd.decomposition._cp import CpAlsEngine
     from tensorly_amd.learned import LearnedExtrapolation, run_extrapolated_sweeps
-    pol = LearnedExtrapolation(net, alpha_max=16.0, warmup=3)
+    if alpha_fixed is not None:
+        pol = LearnedExtrapolation(None, alpha_fixed=alpha_fixed, alpha_max=16.0, warmup=3)
+    else:
+        pol = LearnedExtrapolation(net, alpha_max=16.0, warmup=3)
     eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
     errs, acc, alphas, _ = run_extrapolated_sweeps(eng, n_slots, pol)
     e = errs.cpu().numpy()
@@ -400,22 +436,25 @@ def batched_report(torch, dev, net, log):
     max_sweeps = 1000
     res = {"problems": len(probs), "tensor_bytes": int(b.X.numel() * 4), "tol": 1e-7, "max_sweeps": max_sweeps,
            "autotune_throughput": {"chosen": b.config, "table": table}}
-    policies = [("plain", {}), ("linesearch", {})]
+    # (label, kernel policy, arguments): the fixed step alpha = 4 is the
+    # learned policy's kernel path without a network
+    policies = [("plain", "plain", {}), ("linesearch", "linesearch", {}),
+                ("fixed_alpha4", "learned", {"model": None, "alpha_fixed": 4.0, "alpha_max": 16.0})]
     if net is not None:
-        policies.append(("learned", {"model": net, "alpha_max": 16.0}))
+        policies.append(("learned", "learned", {"model": net, "alpha_max": 16.0}))
     hist = {}
-    for name, kw in policies:
+    for name, pol, kw in policies:
         # launch shape tuned on whole runs of this policy from another random start (seed 1)
-        tt = b.autotune(converge=True, policy=name, seed=1, repeats=1, max_sweeps=max_sweeps, tol=1e-7, **kw)
+        tt = b.autotune(converge=True, policy=pol, seed=1, repeats=1, max_sweeps=max_sweeps, tol=1e-7, **kw)
         out = b.make_outputs(max_sweeps, history=True)
         f = f0.clone()
-        b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)  # warm-up
+        b.launch(f, out, policy=pol, max_sweeps=max_sweeps, tol=1e-7, **kw)  # warm-up
         f.copy_(f0)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        b.launch(f, out, policy=name, max_sweeps=max_sweeps, tol=1e-7, **kw)
+        b.launch(f, out, policy=pol, max_sweeps=max_sweeps, tol=1e-7, **kw)
         e1.record()
         torch.cuda.synchronize()
         sw = out["sweeps"].cpu().numpy()
@@ -452,12 +491,43 @@ def batched_report(torch, dev, net, log):
                                                     "reached": int(sum(
                                                         1 for p in range(len(sw))
                                                         if (he[p, :sw[p]] <= err_p[p] * (1 + 1e-9)).any()))}
+    res["cpu_baseline"] = batched_cpu_baseline(probs)
     return res
 
 
-def cpu_baseline(x32):
-    """Float32 ALS sweeps of the numpy restatement on the host cores: the
-    median over about 10 s of sweeps after a warm-up, and per mode the
+def batched_cpu_baseline(probs, n_sample=48):
+    """The batched workload's CPU baseline: the numpy restatement's parafac
+    (float32 data, random init, tol 1e-7 on |delta rel. error|, at most 1000
+    sweeps) run problem by problem, plain ALS and the reference line search,
+    on the first ``n_sample`` problems; the total for all problems is the
+    sample mean times the problem count."""
+    from threadpoolctl import threadpool_limits
+    from oracle import tensorly_np as T
+    cores = os.cpu_count() or 1
+    out = {"kind": "port", "cores": cores, "problems_timed": n_sample, "problems": len(probs)}
+    with threadpool_limits(cores):
+        for name, ls in (("plain", False), ("linesearch", True)):
+            t0 = time.perf_counter()
+            sweeps = 0
+            for p, (x, r) in enumerate(probs[:n_sample]):
+                (_, _), errs = T.parafac(x, r, n_iter_max=1000, init="random", random_state=p, tol=1e-7,
+                                         linesearch=ls, return_errors=True)
+                sweeps += len(errs)
+            dt = time.perf_counter() - t0
+            out[name] = {"seconds_sample": round(dt, 3),
+                         "seconds_all_problems_estimated": round(dt / n_sample * len(probs), 2),
+                         "mean_recorded_sweeps": round(sweeps / n_sample, 1)}
+    out["sample"] = (f"numpy restatement of the reference parafac, problem by problem, first {n_sample} of "
+                     f"{len(probs)} problems, float32, all {cores} host threads; totals scaled by "
+                     f"{len(probs)}/{n_sample}")
+    return out
+
+
+def cpu_baseline(x32, target=None, max_sweeps=300):
+    """Float32 ALS of the numpy restatement on the host cores from the GPU
+    runs' init (RandomState(1)), sweep by sweep until its relative error
+    reaches ``target`` (the time-to-fit target; at most ``max_sweeps``): the
+    measured time to the target and the median sweep; then per mode the
     median of 3 MTTKRPs after a warm-up."""
     from threadpoolctl import threadpool_limits
     from oracle import tensorly_np as T
@@ -465,6 +535,7 @@ def cpu_baseline(x32):
     rng = np.random.RandomState(1)
     factors = [rng.random_sample((s, RANK)).astype(np.float32) for s in SHAPE]
     w = np.ones(RANK, dtype=np.float32)
+    normx = float(np.linalg.norm(x32.reshape(-1).astype(np.float64)))
 
     def sweep():
         for mode in range(3):
@@ -474,16 +545,20 @@ def cpu_baseline(x32):
                     V = V * (f.T @ f)
             m = T.unfolding_dot_khatri_rao(x32, (w, factors), mode)
             factors[mode] = np.linalg.solve(V.T, m.T).T.astype(np.float32)
-        T.error_calc(x32, 1.0, w, factors, mttkrp=m)
+        return float(T.error_calc(x32, normx, w, factors, mttkrp=m)) / normx
 
+    ts, errs = [], []
+    hit = None
     with threadpool_limits(cores):
-        sweep()  # warm-up
-        ts = []
-        t_start = time.perf_counter()
-        while len(ts) < 3 or (time.perf_counter() - t_start < 10.0 and len(ts) < 200):
+        while len(ts) < max_sweeps:
             t0 = time.perf_counter()
-            sweep()
+            errs.append(sweep())
             ts.append(time.perf_counter() - t0)
+            if target is not None and errs[-1] <= target:
+                hit = len(ts)
+                break
+            if target is None and sum(ts) > 10.0 and len(ts) >= 3:
+                break
         # per-MTTKRP: median of 3 after a warm-up, per mode (unfold copy +
         # materialised Khatri-Rao + np.dot, as in the sweep)
         per_mode = {}
@@ -498,11 +573,13 @@ def cpu_baseline(x32):
     t = float(np.median(ts))
     return {"value": round(3 * MTTKRP_BYTES / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
             "seconds_per_sweep": t, "seconds_per_mttkrp": per_mode,
-            "time_to_fit_seconds_100_sweeps": 100 * t,
-            "sample": f"float32 CP-ALS sweeps (3 MTTKRPs with unfold copy + materialised Khatri-Rao + np.dot, "
-                      f"Hadamard of Grams, np.linalg.solve, fit) of the numpy restatement on the 512^3 tensor: "
-                      f"median of {len(ts)} sweeps after 1 warm-up ({sum(ts):.1f} s of CPU work), "
-                      f"{t:.3f} s/sweep, numpy {np.__version__}, {platform.processor() or 'cpu'}"}
+            "time_to_fit": {"target_error": target, "seconds": float(sum(ts[:hit])) if hit else None,
+                            "sweeps": hit, "final_error": errs[-1]},
+            "sample": f"float32 CP-ALS of the numpy restatement (3 MTTKRPs with unfold copy + materialised "
+                      f"Khatri-Rao + np.dot, Hadamard of Grams, np.linalg.solve, fit) on the 512^3 tensor from "
+                      f"the GPU runs' init, run until its error reached the time-to-fit target: {len(ts)} sweeps "
+                      f"({sum(ts):.1f} s of CPU work), median {t:.3f} s/sweep, numpy {np.__version__}, "
+                      f"{platform.processor() or 'cpu'}"}
 
 
 def main():
@@ -545,6 +622,8 @@ def main():
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed / args.steps * 1e3
     value = 3 * MTTKRP_BYTES * args.steps / elapsed / 1e9
+    sweep_bytes = SWEEP_BYTES_DIMTREE if fast.dimtree else 3 * MTTKRP_BYTES
+    sweep_gbs = sweep_bytes * args.steps / elapsed / 1e9
     log(f"[bench] {args.steps} sweeps: {ms_per_step:.3f} ms/sweep, {value:.1f} GB/s MTTKRP-equivalent "
         f"(dimtree={fast.dimtree})")
 
@@ -566,6 +645,7 @@ def main():
     # launch (mttkrp_stream_kernel, row epilogue) and the largest per sweep
     dom = "mttkrp_partial_modes01_bf16x3"
     dom_bytes = partial_bytes
+    assert dom_bytes == T01_BYTES
     achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
     traffic = measured_traffic()
     log(f"[bench] kernels: {json.dumps(kernels)}")
@@ -573,10 +653,12 @@ def main():
     # ---- time to fit ----------------------------------------------------------------------------------
     ttf = None
     if not args.quick:
-        oracle_path = os.path.join(ROOT, "tests", "golden", "cp512_oracle.npz")
+        oracle_path = os.path.join(ROOT, "tests", "golden", "cp512_converged.npz")
         if os.path.exists(oracle_path):
             ref = np.load(oracle_path)["errors"]
+            # the FP64 oracle run to |delta error| < 1e-9: its converged error + 1e-4
             target = float(ref[-1]) + 1e-4
+            oracle_sweeps = int(np.nonzero(ref <= target)[0][0]) + 1
             res = {}
             for name, prec, dimtree in (("fp32_plain", "fp32", False), ("bf16x3_dimtree", "bf16x3", True)):
                 eng = CpAlsEngine(X, ones, init, precision=prec, dimtree=dimtree, unit_weights=True)
@@ -584,6 +666,9 @@ def main():
                 res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
                 eng.close()
                 log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
+            res["bf16x3_dimtree_fixed_alpha4"] = learned_time_to_fit(X, ones, init, None, args.fit_sweeps, target,
+                                                                     torch, alpha_fixed=4.0)
+            log(f"[bench] time-to-fit bf16x3_dimtree_fixed_alpha4: {res['bf16x3_dimtree_fixed_alpha4']}")
             net = load_model(torch, dev, args.model)
             if net is not None:
                 res["bf16x3_dimtree_learned"] = learned_time_to_fit(X, ones, init, net, args.fit_sweeps, target, torch)
@@ -592,7 +677,8 @@ def main():
                 res["bf16x3_dimtree_learned"] = {"seconds": None, "note": "no trained model file"}
             best = res["bf16x3_dimtree_learned"]["seconds"]
             base = res["fp32_plain"]["seconds"]
-            ttf = {"target_error": target, "oracle_error_100_sweeps": float(ref[-1]), **res,
+            ttf = {"target_error": target, "oracle_converged_error": float(ref[-1]),
+                   "oracle_sweeps_to_target": oracle_sweeps, **res,
                    "speedup_vs_fp32": (base / best) if (base and best) else None,
                    "speedup_plain_bf16x3_vs_fp32": (base / res["bf16x3_dimtree"]["seconds"])
                    if (base and res["bf16x3_dimtree"]["seconds"]) else None}
@@ -605,7 +691,7 @@ def main():
     cpu = None
     if not (args.no_cpu or args.quick):
         log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(x_host)
+        cpu = cpu_baseline(x_host, target=ttf["target_error"] if ttf else None)
         log(f"[bench] CPU baseline {cpu['value']} GB/s")
 
     out = {
@@ -624,6 +710,9 @@ def main():
         "config": {"workload": "cp_512_rank32: 512^3 float32 CP model, rank 32, U[0,1) factors, 1% noise; "
                                "one CP-ALS sweep per step (bf16x3 MFMA MTTKRP, dimension tree, FP64 solve+fit)",
                    "rank": RANK, "shape": list(SHAPE), "sweep": "bf16x3+dimtree", "dimtree_active": fast.dimtree},
+        "sweep": {"algorithmic_bytes": sweep_bytes, "GB_per_s": round(sweep_gbs, 1),
+                  "note": "bytes the sweep itself moves (the dimension tree reads the tensor twice); value "
+                          "counts 3 MTTKRPs of 537,067,520 B"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},

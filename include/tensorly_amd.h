@@ -243,6 +243,89 @@ int tla_hals_nnls(int dtype, void* factor, const void* mttkrp, int64_t rows, int
 
 
 /* ---------------------------------------------------------------------
+ * Masked imputation of parafac(..., mask=...)
+ * (tensorly/decomposition/_cp.py:194-207, :437-445): with the CP model
+ * M = [[w; factors]] (never formed), X_new = X * mask + M * (1 - mask);
+ * sums[0] = ||X_new||^2 and sums[1] = ||X_new - M||^2 (device FP64); X is
+ * overwritten with X_new when `write`.  X and mask are dense C-order in
+ * dtype; factors a host array of ndim device pointers; weights (R values)
+ * nullable; rank <= 64.  Workspace tla_cp_impute_workspace_size bytes.
+ * tla_masked_blend: out = X * mask + low_rank * (1 - mask) elementwise (the
+ * SVD mask imputation of tensorly/tenalg/svd.py:431-439).
+ * ------------------------------------------------------------------- */
+size_t tla_cp_impute_workspace_size(int dtype, int ndim, const int64_t* shape);
+int tla_cp_impute(int dtype, void* X, const void* mask, int ndim, const int64_t* shape,
+                  const void* const* factors, const void* weights, int rank, int write, double* sums,
+                  void* workspace, size_t workspace_bytes, void* stream);
+int tla_masked_blend(int dtype, const void* X, const void* mask, const void* low_rank, void* out, int64_t n,
+                     void* stream);
+
+/* ---------------------------------------------------------------------
+ * Reduced QR of a rows x cols matrix (rows >= cols), the tl.qr that
+ * parafac(..., orthogonalise=...) applies to each factor
+ * (tensorly/decomposition/_cp.py:395-399; numpy: LAPACK geqrf + orgqr).
+ * Householder reflectors with LAPACK's signs (R's diagonal is
+ * -sign(alpha) * norm), FP64 arithmetic, one workgroup.  Q (rows x cols,
+ * may alias A) and R (cols x cols, upper triangular, nullable) in dtype;
+ * workspace tla_qr_workspace_size(rows, cols) bytes.
+ * ------------------------------------------------------------------- */
+size_t tla_qr_workspace_size(int64_t rows, int cols);
+int tla_qr(int dtype, const void* A, int64_t rows, int cols, void* Q, void* R, void* workspace,
+           size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Proximal operators of tensorly/tenalg/proximal.py:153-267
+ * (proximal_operator), for a rows x cols matrix (cols <= 64), by code:
+ * ------------------------------------------------------------------- */
+#define TLA_CON_NONE 0
+#define TLA_CON_NON_NEGATIVE 1        /* clip(v, 0, max(v)) */
+#define TLA_CON_L1 2                  /* soft thresholding by param */
+#define TLA_CON_L2 3                  /* block soft thresholding, Frobenius norm */
+#define TLA_CON_L2_SQUARE 4           /* v / (1 + 2 param) */
+#define TLA_CON_UNIMODALITY 5         /* per column */
+#define TLA_CON_NORMALIZE 6           /* v / max|v| */
+#define TLA_CON_SIMPLEX 7             /* per column, simplex of radius param */
+#define TLA_CON_NORMALIZED_SPARSITY 8 /* hard thresholding, then unit norm */
+#define TLA_CON_SOFT_SPARSITY 9       /* per column, l1 ball of radius param */
+#define TLA_CON_SMOOTHNESS 10         /* per column tridiagonal solve */
+#define TLA_CON_MONOTONICITY 11       /* per column, non-decreasing */
+#define TLA_CON_HARD_SPARSITY 12      /* keep the param largest |v| */
+
+/* out = prox(in) in FP64 arithmetic (in / out in dtype, may alias);
+ * workspace tla_prox_workspace_size(rows, cols) bytes.  Entries of equal
+ * magnitude at the hard-thresholding cut keep the later flat index first. */
+size_t tla_prox_workspace_size(int64_t rows, int cols);
+int tla_prox(int dtype, const void* in, void* out, int64_t rows, int cols, int constraint, double param,
+             void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * ADMM factor update of constrained CP: the inner loop of
+ * tensorly/solvers/admm.py:5-155 as called by constrained_parafac
+ * (tensorly/decomposition/_constrained_cp.py:357-375), in one launch.
+ * With UtU = utu (R x R FP64, device) or, when utu is NULL, the Hadamard
+ * product of the other modes' FP64 Grams (host array of ndim device
+ * pointers, entry `mode` unused), rho = trace(UtU) / R and UtM = mttkrp
+ * (rows x R, dtype), for at most n_iter_max iterations:
+ *   x_split^T = (UtM + rho (x + dual)) (UtU + rho I)^-1
+ *   x         = prox(x_split^T - dual)                (constraint, param)
+ *   dual     += x - x_split^T
+ * stopping when ||x - x_split^T|| < tol ||x|| and ||x - x_old|| < tol ||dual||.
+ * factor (x, rows x R, dtype) is updated in place; x_split (x_split^T,
+ * rows x R FP64) is written; dual (rows x R FP64) is updated.  Device
+ * outputs, each nullable: status_out 0, or 2 when UtU + rho I is singular
+ * (the reference's tl.solve raises; nothing else is written then);
+ * iters_out the iterations run; norms_out[2] = ||x - x_split^T||, ||x|| of
+ * the last iteration (the constraint error of _constrained_cp.py:386-389);
+ * iprod_out = sum(UtM * x) with x as stored (the error's inner product,
+ * _constrained_cp.py:380).  Workspace: tla_admm_workspace_size(rows, R).
+ * ------------------------------------------------------------------- */
+size_t tla_admm_workspace_size(int64_t rows, int rank);
+int tla_admm_update(int dtype, void* factor, double* x_split, double* dual, const void* mttkrp, int64_t rows,
+                    int rank, const double* utu, const double* const* grams, int ndim, int mode, int n_iter_max,
+                    double tol, int constraint, double param, int* status_out, int* iters_out, double* norms_out,
+                    double* iprod_out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
  * Leading left singular vectors through the FP64 Gram (replaces the LAPACK
  * SVD of tensorly/tenalg/svd.py:211-235 on the Tucker and init='svd' paths,
  * called from tensorly/decomposition/_tucker.py:63-79,194-202 and

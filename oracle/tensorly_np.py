@@ -484,8 +484,10 @@ def make_svd_non_negative(tensor, U, S, V, nntype=True):
 
 
 def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True,
-                  u_based_flip_sign=True, non_negative=None, **kwargs):
-    """``tenalg/svd.py:366-447`` without mask imputation (not on the path)."""
+                  u_based_flip_sign=True, non_negative=None, mask=None, n_iter_mask_imputation=5, **kwargs):
+    """``tenalg/svd.py:366-447``.  With ``mask`` (and ``n_eigenvecs``) the
+    missing entries are filled ``n_iter_mask_imputation`` times from the
+    current truncated SVD, U diag(S) V, and the SVD recomputed (``:431-439``)."""
     if method == "truncated_svd":
         fun = truncated_svd
     elif method == "symeig_svd":
@@ -496,6 +498,12 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
         raise ValueError(f"Got svd={method}. However, the possible choices are "
                          "['truncated_svd', 'symeig_svd', 'randomized_svd'] or to pass a callable.")
     U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs)
+    if mask is not None and n_eigenvecs is not None:
+        for _ in range(n_iter_mask_imputation):
+            St = np.eye(U.shape[1], V.shape[0], dtype=matrix.dtype)
+            St[np.arange(S.shape[0]), np.arange(S.shape[0])] = S
+            matrix = matrix * mask + (U @ St @ V) * (1 - mask)
+            U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs)
     if flip_sign:
         U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
     if non_negative is not False and non_negative is not None:
@@ -507,8 +515,9 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
 # CP-ALS  (decomposition/_cp.py)
 # --------------------------------------------------------------------------
 def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=None,
-                  normalize_factors=False, non_negative=False):
-    """``decomposition/_cp.py:26-137`` (no mask)."""
+                  normalize_factors=False, non_negative=False, mask=None, svd_mask_repeats=5):
+    """``decomposition/_cp.py:26-137``; with ``mask`` the SVD init imputes the
+    unfolded mask's missing entries (``:66-76``)."""
     rng = check_random_state(random_state)
     if isinstance(init, str) and init == "random":
         weights, factors = random_cp(tensor.shape, rank, normalise_factors=False,
@@ -517,7 +526,9 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=No
         factors = []
         for mode in range(tensor.ndim):
             U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd,
-                                    non_negative=non_negative)
+                                    non_negative=non_negative,
+                                    mask=None if mask is None else unfold(mask, mode),
+                                    n_iter_mask_imputation=svd_mask_repeats)
             if mode == 0:
                 k = min(rank, S.shape[0])
                 U = U.copy()
@@ -544,11 +555,19 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", random_state=No
     return weights, factors
 
 
-def error_calc(tensor, norm_tensor, weights, factors, mttkrp=None):
-    """``decomposition/_cp.py:162-227`` (no mask / sparsity).
+def error_calc(tensor, norm_tensor, weights, factors, mttkrp=None, mask=None):
+    """``decomposition/_cp.py:162-227`` (no sparsity).
 
     With the last mode's MTTKRP: ``sqrt(|‖X‖² + ‖M‖² − 2<X, M>|)`` where
-    ``<X, M> = sum(mttkrp * A_last)``; otherwise the full reconstruction."""
+    ``<X, M> = sum(mttkrp * A_last)``; otherwise the full reconstruction.
+    With ``mask`` the tensor's missing entries are first replaced by the
+    model and its norm recomputed; returns ``(error, tensor, norm_tensor)``
+    then."""
+    if mask is not None:
+        rec = cp_to_tensor((weights, factors))
+        tensor = tensor * mask + rec * (1 - mask)
+        norm_tensor = norm(tensor, 2)
+        return norm(tensor - rec, 2), tensor, norm_tensor
     if mttkrp is None:
         rec = cp_to_tensor((weights, factors))
         return norm(tensor - rec, 2)
@@ -561,8 +580,10 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
             normalize_factors=False, orthogonalise=False, tol=1e-8, random_state=None,
             verbose=0, return_errors=False, l2_reg=0, cvg_criterion="abs_rec_error",
             fixed_modes=None, linesearch=False, callback=None, mttkrp_fn=None,
-            trace=None):
-    """``decomposition/_cp.py:230-543`` (no mask / sparsity).
+            trace=None, mask=None, svd_mask_repeats=5):
+    """``decomposition/_cp.py:230-543`` (no sparsity).  With ``mask`` the
+    working tensor's missing entries are re-imputed from the model after
+    every sweep (``:437-445``) and the error is that of the imputed tensor.
 
     ``mttkrp_fn`` swaps the MTTKRP (default: the materialised-KR form of
     ``mttkrp.py:47-50``); ``trace``, if a list, receives per-iteration
@@ -575,7 +596,8 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
         acc_pow, acc_fail, max_fail = 2.0, 0, 4
     weights, factors = initialize_cp(tensor, rank, init=init, svd=svd,
                                      random_state=random_state,
-                                     normalize_factors=normalize_factors)
+                                     normalize_factors=normalize_factors, mask=mask,
+                                     svd_mask_repeats=svd_mask_repeats)
     factors = list(factors)
     rec_errors = []
     norm_tensor = norm(tensor, 2)
@@ -589,7 +611,10 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
     modes_list = [m for m in range(tensor.ndim) if m not in fixed_modes]
 
     if callback is not None:
-        err0 = error_calc(tensor, norm_tensor, weights, factors)
+        if mask is not None:
+            err0, _, norm_tensor = error_calc(tensor, norm_tensor, weights, factors, mask=mask)
+        else:
+            err0 = error_calc(tensor, norm_tensor, weights, factors)
         callback((weights, factors), err0 / norm_tensor)
 
     rec_error = None
@@ -611,24 +636,37 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd",
                                                          np.transpose(mttkrp)))
         line_iter = bool(linesearch and iteration % 2 == 0 and iteration > 5)
         if (tol or return_errors) and not line_iter:
-            unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
+            if mask is not None:
+                unnorm, tensor, norm_tensor = error_calc(tensor, norm_tensor, weights, factors, mask=mask)
+            else:
+                unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
+        elif mask is not None:
+            tensor = tensor * mask + cp_to_tensor((weights, factors)) * (1 - mask)
         rec = {"iteration": iteration}
         if line_iter:
             jump = iteration ** (1.0 / acc_pow)
             new_w = weights_last + (weights - weights_last) * jump
             new_f = [factors_last[i] + (factors[i] - factors_last[i]) * jump
                      for i in range(tensor.ndim)]
-            new_err = error_calc(tensor, norm_tensor, new_w, new_f)
+            if mask is not None:
+                new_err, new_tensor, new_norm = error_calc(tensor, norm_tensor, new_w, new_f, mask=mask)
+            else:
+                new_err = error_calc(tensor, norm_tensor, new_w, new_f)
+                new_tensor, new_norm = tensor, norm_tensor
             rec["jump"] = jump
-            rec["new_error"] = new_err / norm_tensor
+            rec["new_error"] = new_err / new_norm
             rec["last_error"] = rec_errors[-1]
-            if new_err / norm_tensor < rec_errors[-1]:
+            if new_err / new_norm < rec_errors[-1]:
                 factors, weights = new_f, new_w
+                tensor, norm_tensor = new_tensor, new_norm
                 unnorm = new_err
                 acc_fail = 0
                 rec["accepted"] = True
             else:
-                unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
+                if mask is not None:
+                    unnorm, tensor, norm_tensor = error_calc(tensor, norm_tensor, weights, factors, mask=mask)
+                else:
+                    unnorm = error_calc(tensor, norm_tensor, weights, factors, mttkrp)
                 acc_fail += 1
                 rec["accepted"] = False
                 if acc_fail == max_fail:

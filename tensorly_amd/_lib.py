@@ -61,6 +61,16 @@ _SIGNATURES = {
     "tla_mttkrp_coo_workspace_size": (c_sz, [c_i, c_i64, c_i]),
     "tla_mttkrp_coo": (c_i, [c_i, c_i64, c_i, P_i64, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_i64, PP, c_p, c_i,
                              c_i, c_p, c_p, c_sz, c_p]),
+    "tla_cp_impute_workspace_size": (c_sz, [c_i, c_i, P_i64]),
+    "tla_cp_impute": (c_i, [c_i, c_p, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_p, c_p, c_sz, c_p]),
+    "tla_masked_blend": (c_i, [c_i, c_p, c_p, c_p, c_p, c_i64, c_p]),
+    "tla_qr_workspace_size": (c_sz, [c_i64, c_i]),
+    "tla_qr": (c_i, [c_i, c_p, c_i64, c_i, c_p, c_p, c_p, c_sz, c_p]),
+    "tla_prox_workspace_size": (c_sz, [c_i64, c_i]),
+    "tla_prox": (c_i, [c_i, c_p, c_p, c_i64, c_i, c_i, c_d, c_p, c_sz, c_p]),
+    "tla_admm_workspace_size": (c_sz, [c_i64, c_i]),
+    "tla_admm_update": (c_i, [c_i, c_p, c_p, c_p, c_p, c_i64, c_i, c_p, PP, c_i, c_i, c_i, c_d, c_i, c_d, c_p, c_p, c_p,
+                              c_p, c_p, c_sz, c_p]),
     "tla_mode_gram64_workspace_size": (c_sz, [c_i, P_i64, c_i]),
     "tla_mode_gram64": (c_i, [c_i, c_p, c_i, P_i64, c_i, c_p, c_p, c_sz, c_p]),
     "tla_sym_eig_workspace_size": (c_sz, [c_i]),

@@ -1246,6 +1246,17 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
   for (int m = 0; m < ndim; ++m) TLA_REQUIRE(shape[m] >= 1, "empty mode %d", m);
   *p = Plan{};
   const bool standard = (ke == kb + 1);
+  // the MFMA paths size their grids in 32-bit counts of 16-row / 64-column
+  // tiles; beyond that a standard MTTKRP takes the generic kernel and a
+  // partial one is refused
+  constexpr int64_t kMaxTiles = 0x7fffffff;
+  auto too_big = [&](int64_t tiles) -> int {
+    TLA_REQUIRE(standard, "partial contraction: %lld tiles exceed the grid limit", (long long)tiles);
+    TLA_REQUIRE(x_dtype != TLA_BF16, "bf16 tensor copy: %lld tiles exceed the grid limit", (long long)tiles);
+    p->path = Path::Generic;
+    p->ws_bytes = 0;
+    return TLA_OK;
+  };
   const bool mfma = prec_ok(x_dtype, precision) && R <= 64;
   const int RT = (R + 15) / 16;
   p->RT = RT;
@@ -1271,6 +1282,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     const int64_t J = shape[ndim - 2];
     int64_t Prow = 1;
     for (int m = 0; m < ndim - 1; ++m) Prow *= shape[m];
+    if ((Prow + 15) / 16 > kMaxTiles || (In + 15) / 16 > kMaxTiles) return too_big(std::max(Prow, In) / 16);
     {
       // Persistent interleaved kernel when F_k fits LDS whole.
       const size_t lds = (precision == TLA_PREC_FP32) ? size_t(RT * 16) * (K + 4) * 4
@@ -1371,6 +1383,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
       p->ws_bytes = align256(p->partial_elems * 4);
       return TLA_OK;
     }
+    // more K panels than a grid dimension holds: the generic kernel
+    if ((K + 127) / 128 > 65535) return too_big((K + 127) / 128);
     p->path = Path::Inner;
     InnerArgs& a = p->in;
     a.K = K; a.In = In; a.Pa = Pa; a.Pb = Pb;
@@ -1404,6 +1418,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
   int64_t P = 1;
   for (int m = 0; m < kb; ++m) P *= shape[m];
   p->out_rows = Kc;
+  if ((Kc + 63) / 64 > kMaxTiles / 8 && mfma) return too_big((Kc + 63) / 64);
   const int64_t J = kb >= 1 ? shape[kb - 1] : 1;
   const bool shape_ok = Kc % 4 == 0 && kb >= 1 &&
                         (precision == TLA_PREC_FP32 || J % 8 == 0);
@@ -1635,6 +1650,10 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
                     const InvJob* job = nullptr, bool* job_taken = nullptr) {
   Plan p = p0;
   if (job_taken) *job_taken = false;
+  // A matrix's MTTKRP multiplies by the one other factor alone: khatri_rao
+  // of a single matrix returns it without the weights
+  // (tensorly/tenalg/core_tenalg/_khatri_rao.py:67-69).
+  if (ndim == 2 && ke == kb + 1) weights = nullptr;
   // the side job rides along as one extra workgroup when the grid leaves a
   // slot of the 256 free (all workgroups resident at once)
   if (job && job->R > 0 && (p.path == Path::Stream || p.path == Path::LastStream) && p.grid.x + 1 <= 256) {
@@ -1664,7 +1683,8 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     s.Fk = (const float*)factors[ndim - 1];
     float* sqbuf = (float*)ws;
     float* part = (float*)((char*)ws + align256(size_t(p.gm_rows) * R * 4));
-    if (p.sepi != SE_ROWS) {
+    const bool rows_out = p.sepi == SE_ROWS;
+    if (!rows_out) {
       GmArgs g = p.gm;
       if (p.sepi == SE_ACCJ) {
         for (int m = 0; m < g.n; ++m) g.fac[m] = (const float*)factors[m];
@@ -1680,12 +1700,12 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
       TLA_LAUNCH_CHECK();
     }
     s.Sq = sqbuf;
-    s.out = (p.sepi == SE_ROWS) ? (float*)out : part;
+    s.out = rows_out ? (float*)out : part;
     if (x_dtype == TLA_BF16) rc = launch_stream<TLA_PREC_BF16, __bf16>(p, st);
     else if (precision == TLA_PREC_FP32) rc = launch_stream<TLA_PREC_FP32, float>(p, st);
     else if (precision == TLA_PREC_BF16) rc = launch_stream<TLA_PREC_BF16, float>(p, st);
     else rc = launch_stream<TLA_PREC_BF16X3, float>(p, st);
-    if (rc != TLA_OK || p.sepi == SE_ROWS) return rc;
+    if (rc != TLA_OK || rows_out) return rc;
     if (p.sepi == SE_REDQ) {
       const int64_t n = p.t_In * R;
       if (p.t_Pa == 1)
@@ -1797,10 +1817,6 @@ int tla::mttkrp_launch(int x_dtype, const void* X, int ndim, const int64_t* shap
   if (job_taken) *job_taken = false;
   TLA_REQUIRE(X && out && factors, "NULL pointer argument");
   if (x_dtype == TLA_F64 && precision != TLA_PREC_VALU) precision = TLA_PREC_VALU;
-  // A matrix's MTTKRP multiplies by the one other factor alone: khatri_rao
-  // of a single matrix returns it without the weights
-  // (tensorly/tenalg/core_tenalg/_khatri_rao.py:67-69).
-  if (ndim == 2 && ke == kb + 1) weights = nullptr;
   Plan p;
   int rc = make_plan(x_dtype, ndim, shape, rank, kb, ke, precision, &p, job != nullptr && job->R > 0);
   if (rc != TLA_OK) return rc;

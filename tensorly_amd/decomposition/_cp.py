@@ -13,8 +13,16 @@ Extensions (keyword-only, after the reference arguments):
 contractions between modes, orders 3 and 4) and ``extrapolation``
 (``None``, or a :class:`tensorly_amd.learned.LearnedExtrapolation`).
 
+``orthogonalise`` replaces each factor by the Q of its Householder QR
+(``tla_qr``) before the sweeps it names (``_cp.py:330-331, 395-399``).
+
+``mask`` keeps a working copy of the tensor whose missing entries are
+re-imputed from the model after every sweep by one fused pass
+(``tla_cp_impute``), which also gives the error (``_cp.py:194-207,
+437-445``); ``init='svd'`` imputes the unfolding ``svd_mask_repeats`` times.
+
 Not implemented (raise NotImplementedError rather than run elsewhere):
-``mask``, ``sparsity``, ``orthogonalise``, complex tensors.
+``sparsity``, complex tensors.
 """
 import warnings
 
@@ -45,9 +53,9 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", non_negative=Fa
     """``_cp.py:26-137``.  ``init='random'`` draws ``random_sample((I_m, R))``
     per mode in order from the RandomState (``random/base.py:103-105``) and
     casts to the tensor's dtype on its device; ``init='svd'`` uses the GPU
-    Gram eigensolver (:func:`tensorly_amd.svd.svd_interface`)."""
-    if mask is not None:
-        raise NotImplementedError("mask is not supported by tensorly_amd.parafac")
+    Gram eigensolver (:func:`tensorly_amd.svd.svd_interface`), with the
+    mask's missing entries imputed ``svd_mask_repeats`` times when a mask is
+    given (``:66-76``)."""
     rng = _check_random_state(random_state)
     dt, dev = tensor.dtype, tensor.device
     if isinstance(init, str) and init == "random":
@@ -58,7 +66,9 @@ def initialize_cp(tensor, rank, init="svd", svd="truncated_svd", non_negative=Fa
         from ..base import unfold
         factors = []
         for mode in range(tensor.dim()):
-            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd, non_negative=non_negative)
+            U, S, _ = svd_interface(unfold(tensor, mode), n_eigenvecs=rank, method=svd, non_negative=non_negative,
+                                    mask=None if mask is None else unfold(mask, mode),
+                                    n_iter_mask_imputation=svd_mask_repeats)
             if mode == 0:
                 k = min(rank, S.shape[0])
                 U = U.clone()
@@ -109,12 +119,20 @@ def _cp_norm2_64(weights, factors):
 
 
 def error_calc(tensor, norm_tensor, weights, factors, sparsity=None, mask=None, mttkrp=None):
-    """``_cp.py:162-227`` (no sparsity / mask).  Both branches use
+    """``_cp.py:162-227`` (no sparsity).  Both branches use
     ``sqrt(|‖X‖² + ‖M‖² − 2<X, M>|)``; without ``mttkrp`` the inner product
     comes from one fused MTTKRP pass instead of a full reconstruction (same
-    quantity).  Returns ``(unnormalised error, tensor, norm_tensor)``."""
-    if sparsity or mask is not None:
-        raise NotImplementedError("sparsity / mask are not supported by tensorly_amd")
+    quantity).  With ``mask`` the tensor's missing entries are replaced by
+    the model first (a new tensor) and its norm recomputed, in one
+    ``tla_cp_impute`` pass.  Returns ``(unnormalised error, tensor,
+    norm_tensor)``."""
+    if sparsity:
+        raise NotImplementedError("sparsity is not supported by tensorly_amd")
+    if mask is not None:
+        _lib.require_on_device(mask, tensor.device, "mask")
+        work = tensor.contiguous().clone()
+        err, nrm = _Imputer(work, mask.to(tensor.dtype).contiguous()).run(weights, factors, write=True)
+        return err, work, nrm
     nm2 = _cp_norm2_64(weights, factors)
     if mttkrp is None:
         ip = _inner_with_last_mttkrp(tensor, weights, factors)
@@ -216,6 +234,33 @@ class CpAlsEngine:
             pass
 
 
+class _Imputer:
+    """The working tensor of ``parafac(..., mask=...)``: after each sweep its
+    missing entries are replaced by the model (``_cp.py:194-207, 437-445``)
+    by one ``tla_cp_impute`` pass, which also gives the imputed tensor's
+    norm and the reconstruction error."""
+
+    def __init__(self, work, mask):
+        self.work, self.mask, self.dev = work, mask, work.device
+        self._sh, self._keep_sh = shape_array(work.shape)
+        self.nb = lib().tla_cp_impute_workspace_size(_lib.dtype_code(work), work.dim(), self._sh)
+        self.ws = _lib.workspace(self.nb, self.dev)
+        self.sums = torch.zeros(2, dtype=torch.float64, device=self.dev)
+
+    def run(self, weights, factors, write):
+        """Impute from the model ``(weights, factors)`` (``write``: into the
+        working tensor); returns device ``(error, norm)`` of the imputed
+        tensor."""
+        facs = [f.to(self.work.dtype).contiguous() for f in factors]
+        w = weights.to(self.work.dtype).contiguous()
+        pp, _keep = ptr_array(facs)
+        check(lib().tla_cp_impute(_lib.dtype_code(self.work), ptr(self.work), ptr(self.mask), self.work.dim(),
+                                  self._sh, pp, ptr(w), facs[0].shape[1], int(bool(write)), ptr(self.sums),
+                                  ptr(self.ws), self.nb, stream(self.dev)), "cp_impute")
+        s = torch.sqrt(self.sums)
+        return s[1], s[0]
+
+
 def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", normalize_factors=False,
             orthogonalise=False, tol=1e-8, random_state=None, verbose=0, return_errors=False,
             sparsity=None, l2_reg=0, mask=None, cvg_criterion="abs_rec_error", fixed_modes=None,
@@ -225,19 +270,33 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", norma
 
     Returns ``CPTensor((weights, factors))`` and, with ``return_errors``, the
     list of relative reconstruction errors (Python floats)."""
-    if mask is not None or sparsity:
-        raise NotImplementedError("mask / sparsity are not supported by tensorly_amd.parafac")
-    if orthogonalise:
-        raise NotImplementedError("orthogonalise is not supported by tensorly_amd.parafac")
+    if sparsity:
+        raise NotImplementedError("sparsity is not supported by tensorly_amd.parafac")
+    if orthogonalise and not isinstance(orthogonalise, int):
+        orthogonalise = n_iter_max  # (True is an int: iterations 0 and 1, as in the reference)
     _lib.require_gpu_tensor(tensor, "tensor")
     if tensor.dtype not in (torch.float32, torch.float64):
         raise TypeError("parafac needs a float32 or float64 tensor")
     rank = validate_cp_rank(tuple(tensor.shape), rank=rank)
     if linesearch and extrapolation is not None:
         raise ValueError("choose either the reference line search or a learned extrapolation")
+    if orthogonalise and extrapolation is not None:
+        raise NotImplementedError("orthogonalise runs the reference's host loop, not the extrapolated driver")
 
+    imputer = None
+    if mask is not None:
+        _lib.require_on_device(mask, tensor.device, "mask")
+        if tuple(mask.shape) != tuple(tensor.shape):
+            raise ValueError(f"mask has shape {tuple(mask.shape)}, the tensor {tuple(tensor.shape)}")
+        if extrapolation is not None or tensor_bf16 is not None:
+            raise NotImplementedError("mask runs the reference's host loop on a float32 / float64 working copy")
+        mask = mask.to(tensor.dtype).contiguous()
     weights, factors = initialize_cp(tensor, rank, init=init, svd=svd, random_state=random_state,
-                                     normalize_factors=normalize_factors)
+                                     normalize_factors=normalize_factors, mask=mask,
+                                     svd_mask_repeats=svd_mask_repeats)
+    if mask is not None:
+        tensor = tensor.contiguous().clone()  # the working copy whose missing entries are imputed
+        imputer = _Imputer(tensor, mask)
     if weights is None:
         weights = torch.ones(rank, dtype=tensor.dtype, device=tensor.device)
     nd = tensor.dim()
@@ -258,21 +317,34 @@ def parafac(tensor, rank, n_iter_max=100, init="svd", svd="truncated_svd", norma
             return run_extrapolated_als(eng, n_iter_max, tol, extrapolation, return_errors,
                                         normalize_factors, cvg_criterion, callback, verbose)
         return _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize_factors, cvg_criterion,
-                          linesearch, callback, verbose)
+                          linesearch, callback, verbose, orthogonalise=orthogonalise, imputer=imputer)
     finally:
         eng.close()
 
 
+def _orthogonalise(eng):
+    """``_cp.py:395-399``: every factor with at least ``rank`` rows replaced
+    by the Q of its reduced QR (Householder, LAPACK's signs)."""
+    from ..svd import qr
+    rank = eng.rank
+    eng.set_factors(eng.weights, [qr(f)[0] if min(f.shape) >= rank else f for f in eng.factors])
+
+
 def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_criterion, linesearch, callback,
-               verbose):
+               verbose, orthogonalise=False, imputer=None):
     dev = tensor.device
     errs_dev = torch.zeros(max(1, n_iter_max), dtype=torch.float64, device=dev)
     want_err = bool(tol or return_errors)
-    host_loop = bool(tol) or callback is not None or linesearch or verbose
+    # the device-side sweep loop normalises float32 factors only
+    host_loop = (bool(tol) or callback is not None or linesearch or verbose or bool(orthogonalise)
+                 or imputer is not None or (normalize and tensor.dtype != torch.float32))
     norm_tensor = torch.sqrt(eng.normx2())
 
     if callback is not None:
-        e0, _, _ = error_calc(tensor, norm_tensor, eng.weights, eng.factors)
+        if imputer is not None:
+            e0, norm_tensor = imputer.run(eng.weights, eng.factors, write=False)
+        else:
+            e0, _, _ = error_calc(tensor, norm_tensor, eng.weights, eng.factors)
         callback(CPTensor((eng.weights.clone(), [f.clone() for f in eng.factors])), float(e0 / norm_tensor))
 
     if not host_loop:
@@ -287,23 +359,40 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
         acc_pow, acc_fail, max_fail = 2.0, 0, 4
     rec_error = None
     for iteration in range(n_iter_max):
+        if orthogonalise and iteration <= orthogonalise:
+            _orthogonalise(eng)
         if linesearch and iteration % 2 == 0:
             factors_last = [f.clone() for f in eng.factors]
             weights_last = eng.weights.clone()
-        eng.sweeps(1, errs_dev, iteration, False)
+        eng.sweeps(1, errs_dev if imputer is None else None, iteration, False)
         eng.raise_if_singular()
         line_iter = bool(linesearch and iteration % 2 == 0 and iteration > 5)
+        if imputer is not None:
+            # the working tensor is re-imputed from the updated model whether
+            # or not the error is wanted (_cp.py:437-445)
+            unnorm, norm_i = imputer.run(eng.weights, eng.factors, write=True)
+            if want_err and not line_iter:
+                norm_tensor = norm_i
         if line_iter:
             jump = iteration ** (1.0 / acc_pow)
             new_w = weights_last + (eng.weights - weights_last) * jump
             new_f = [fl + (f - fl) * jump for f, fl in zip(eng.factors, factors_last)]
-            new_err, _, _ = error_calc(tensor, norm_tensor, new_w, new_f)
-            if float(new_err / norm_tensor) < rec_errors[-1]:
+            if imputer is not None:
+                new_err, new_norm = imputer.run(new_w, new_f, write=False)
+            else:
+                new_err, _, _ = error_calc(tensor, norm_tensor, new_w, new_f)
+                new_norm = norm_tensor
+            if float(new_err / new_norm) < rec_errors[-1]:
                 eng.set_factors(new_w, new_f)
+                if imputer is not None:
+                    imputer.run(new_w, new_f, write=True)
+                    norm_tensor = new_norm
                 acc_fail = 0
                 if verbose:
                     print(f"Accepted line search jump of {jump}.")
             else:
+                if imputer is not None:
+                    _, norm_tensor = imputer.run(eng.weights, eng.factors, write=True)
                 acc_fail += 1
                 if verbose:
                     print(f"Line search failed for jump of {jump}.")
@@ -311,7 +400,7 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
                     acc_pow += 1.0
                     acc_fail = 0
         elif want_err:
-            rec_error = float(errs_dev[iteration])
+            rec_error = float(unnorm / norm_tensor) if imputer is not None else float(errs_dev[iteration])
             rec_errors.append(rec_error)
         if callback is not None:
             cp = CPTensor((eng.weights.clone(), [f.clone() for f in eng.factors]))

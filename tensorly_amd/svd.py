@@ -15,7 +15,7 @@ import torch
 from . import _lib
 from ._lib import check, lib, ptr, shape_array, stream, workspace
 
-__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd_non_negative", "gram_eigh",
+__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd_non_negative", "gram_eigh", "qr",
            "mode_gram", "SVD_FUNS"]
 
 SVD_FUNS = ["truncated_svd", "symeig_svd"]
@@ -148,16 +148,28 @@ def make_svd_non_negative(tensor, U, S, V, nntype=True):
 
 def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=True, u_based_flip_sign=True,
                   non_negative=None, mask=None, n_iter_mask_imputation=5, **kwargs):
-    """``svd.py:366-447`` (no mask imputation); ``non_negative`` applies
+    """``svd.py:366-447``.  With ``mask`` (and ``n_eigenvecs``) the missing
+    entries are filled ``n_iter_mask_imputation`` times from the current
+    truncated SVD, U diag(S) V (two MFMA products), by ``tla_masked_blend``,
+    and the SVD recomputed (``:431-439``); ``non_negative`` applies
     :func:`make_svd_non_negative` after the sign flip, as the reference."""
-    if mask is not None:
-        raise NotImplementedError("mask imputation is not supported by tensorly_amd")
     if method not in ("truncated_svd", "symeig_svd") and not callable(method):
         raise ValueError(f"Got svd={method}. However, the possible choices are {SVD_FUNS} or to pass a callable.")
-    if callable(method):
-        U, S, V = method(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
-    else:
-        U, S, V = truncated_svd(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
+    fun = method if callable(method) else truncated_svd
+    U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
+    if mask is not None and n_eigenvecs is not None:
+        from .tenalg import mode_dot
+        _lib.require_on_device(mask, matrix.device, "mask")
+        m = mask.to(matrix.dtype).contiguous()
+        for _ in range(n_iter_mask_imputation):
+            k = S.shape[0]
+            low = mode_dot((V[:k] * S[:, None]).contiguous(), U[:, :k].contiguous(), 0)
+            blended = torch.empty_like(low)
+            x = matrix.contiguous()
+            check(lib().tla_masked_blend(_lib.dtype_code(x), ptr(x), ptr(m), ptr(low), ptr(blended), x.numel(),
+                                         stream(x.device)), "masked_blend")
+            matrix = blended
+            U, S, V = fun(matrix, n_eigenvecs=n_eigenvecs, **kwargs)
     if flip_sign:
         U, V = svd_flip(U, V, u_based_decision=u_based_flip_sign)
     if non_negative is not False and non_negative is not None:
@@ -175,3 +187,25 @@ def leading_left_singular_vectors(tensor, mode, k, warm=None):
     U, S = _left_svd_from_gram(G, n, min(n, q), k, warm=warm)
     U, _ = svd_flip(U, None)
     return U.to(tensor.dtype), S.to(tensor.dtype)
+
+
+def qr(matrix):
+    """Reduced QR ``(Q, R)`` of a tall GPU matrix (rows >= cols): ``tl.qr``
+    of the numpy backend (``backend/numpy_backend.py:75``, ``np.linalg.qr``,
+    LAPACK geqrf + orgqr) with the same Householder signs, in one
+    ``tla_qr`` launch (FP64 arithmetic)."""
+    _lib.require_gpu_tensor(matrix, "matrix")
+    if matrix.dim() != 2:
+        raise ValueError(f"qr takes a matrix, got {matrix.dim()} dimensions")
+    if matrix.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"qr takes float32 / float64 matrices, got {matrix.dtype}")
+    A = matrix.contiguous()
+    rows, cols = A.shape
+    if rows < cols:
+        raise NotImplementedError("qr of a wide matrix (rows < cols) is not supported")
+    Q = torch.empty_like(A)
+    R = torch.empty((cols, cols), dtype=A.dtype, device=A.device)
+    nb = lib().tla_qr_workspace_size(rows, cols)
+    ws = workspace(nb, A.device)
+    check(lib().tla_qr(_lib.dtype_code(A), ptr(A), rows, cols, ptr(Q), ptr(R), ptr(ws), nb, stream(A.device)), "qr")
+    return Q, R

@@ -90,7 +90,48 @@ def test_host_validation_matches_reference_errors():
 
 
 def test_unsupported_features_raise():
+    """sparsity has no GPU path and raises before any device work; a host
+    tensor is refused as such."""
     import tensorly_amd as tla
     X = torch.zeros((3, 4, 5))
     with pytest.raises(NotImplementedError):
+        tla.parafac(X, 2, sparsity=0.1)
+    with pytest.raises(TypeError, match="GPU"):
         tla.parafac(X, 2, mask=torch.ones_like(X))
+
+
+def test_planner_survives_extreme_shapes():
+    """The host-side planners (workspace queries) on random shapes, ranks,
+    dtypes and precisions, up to 2^40 entries and mode sizes of 2^24: no
+    host crash (a 32-bit grid overflow once divided by zero).  Runs in a
+    child process so that a crash fails this test instead of the session."""
+    import subprocess
+    import sys
+    code = r'''
+import sys
+sys.path.insert(0, %r)
+import numpy as np
+from tensorly_amd import _lib
+L = _lib.lib()
+rng = np.random.default_rng(0)
+vals = [1, 2, 3, 4, 7, 8, 15, 16, 17, 31, 32, 33, 48, 63, 64, 65, 127, 128, 129, 255, 256, 257, 511, 512, 513,
+        1000, 1024, 4096, 65536, 1 << 20, 1 << 24]
+for trial in range(1500):
+    nd = int(rng.integers(2, 6))
+    shape = [int(rng.choice(vals)) for _ in range(nd)]
+    if np.prod(np.array(shape, dtype=np.float64)) > 2 ** 40:
+        continue
+    R = int(rng.choice([1, 5, 16, 17, 32, 33, 48, 64, 65]))
+    sh, _k = _lib.shape_array(shape)
+    for dt in (0, 1, 2):
+        for prec in (0, 1, 2, 3):
+            for m in range(nd):
+                L.tla_mttkrp_workspace_size(dt, nd, sh, R, m, prec)
+            for kb, ke in [(0, k) for k in range(1, nd)] + [(k, nd) for k in range(1, nd)]:
+                L.tla_mttkrp_partial_supported(dt, nd, sh, R, kb, ke, prec)
+                L.tla_mttkrp_partial_workspace_size(dt, nd, sh, R, kb, ke, prec)
+            L.tla_cp_als_workspace_size(dt, nd, sh, R, prec, 1)
+print("ok")
+''' % ROOT
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0 and "ok" in res.stdout, (res.returncode, res.stderr[-2000:])

@@ -311,20 +311,117 @@ def test_parafac_coo_singular_system_raises(cuda):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_parafac_matrix_normalize_factors(cuda, dtype):
     """Order 2 with normalize_factors=True: the weights scale V but not the
-    MTTKRP (khatri_rao of one matrix is unweighted, ``_khatri_rao.py:67-69``),
-    so the drop-in must follow the oracle's iterates and errors exactly."""
+    MTTKRP (khatri_rao of one matrix is unweighted, ``_khatri_rao.py:67-69``).
+    The first sweep (unit weights) and its normalisation match the oracle.
+    From then on the reference's iteration is unstable: the second sweep's
+    normal equations have condition numbers of 1e9 and 1e16, so its values are
+    set by rounding (numpy's LU and the GPU's elimination differ by tens of
+    percent) and both errors jump from 0.79 to above 1e4."""
     import tensorly_amd as tla
     rng = np.random.default_rng(33)
     M = rng.random((30, 4)) @ rng.random((4, 45)) + 0.01 * rng.standard_normal((30, 45))
-    (w_ref, f_ref), e_ref = T.parafac(M, 4, n_iter_max=15, init="random", random_state=2, tol=0,
+    (w_ref, f_ref), e_ref = T.parafac(M, 4, n_iter_max=1, init="random", random_state=2, tol=0,
                                       normalize_factors=True, return_errors=True)
-    cp, errs = tla.parafac(g(M, cuda, dtype), 4, n_iter_max=15, init="random", random_state=2, tol=0,
+    cp, errs = tla.parafac(g(M, cuda, dtype), 4, n_iter_max=1, init="random", random_state=2, tol=0,
                            normalize_factors=True, return_errors=True)
-    if dtype == torch.float64:
-        np.testing.assert_allclose(errs, e_ref, rtol=1e-9)
-        np.testing.assert_allclose(cp.weights.cpu().numpy(), w_ref, rtol=1e-8)
-        for a, b in zip(cp.factors, f_ref):
-            np.testing.assert_allclose(a.cpu().numpy(), b, rtol=1e-7, atol=1e-10)
-    else:
-        d = np.abs(np.array(errs) ** 2 - np.array(e_ref) ** 2)
-        assert np.max(d) < 1e-6, d
+    rtol = 1e-9 if dtype == torch.float64 else 1e-3
+    np.testing.assert_allclose(errs, e_ref, rtol=rtol)
+    np.testing.assert_allclose(cp.weights.cpu().numpy(), w_ref, rtol=rtol)
+    for a, b in zip(cp.factors, f_ref):
+        np.testing.assert_allclose(a.cpu().numpy(), b, rtol=10 * rtol, atol=10 * rtol)
+        np.testing.assert_allclose(np.linalg.norm(a.cpu().numpy(), axis=0), 1.0, rtol=rtol)
+    # the second sweep's first V = (A_1^T A_1) * w w^T is already badly
+    # conditioned (its second one near 1e16)
+    V = (f_ref[1].T @ f_ref[1]) * np.outer(w_ref, w_ref)
+    assert np.linalg.cond(V) > 1e8
+    _, e2_ref = T.parafac(M, 4, n_iter_max=2, init="random", random_state=2, tol=0, normalize_factors=True,
+                          return_errors=True)
+    _, e2 = tla.parafac(g(M, cuda, dtype), 4, n_iter_max=2, init="random", random_state=2, tol=0,
+                        normalize_factors=True, return_errors=True)
+    assert e2_ref[1] > 1e4 and e2[1] > 1e4
+
+
+@pytest.mark.parametrize("shape", [(10, 3), (64, 64), (300, 17), (1000, 48), (5, 5)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_qr_matches_numpy_signs(cuda, shape, dtype):
+    """tla_qr against numpy's qr (LAPACK geqrf + orgqr): the same Q and R,
+    signs included, and Q R = A."""
+    from tensorly_amd.svd import qr
+    rng = np.random.default_rng(shape[0] + shape[1])
+    A = rng.standard_normal(shape)
+    Q, R = qr(g(A, cuda, dtype))
+    Qr, Rr = np.linalg.qr(A)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    np.testing.assert_allclose(Q.cpu().numpy(), Qr, atol=tol * 10)
+    np.testing.assert_allclose(R.cpu().numpy(), Rr, atol=tol * 10 * np.abs(Rr).max())
+    np.testing.assert_allclose((Q.double() @ R.double()).cpu().numpy(), A, atol=tol * 10 * np.abs(A).max())
+
+
+@pytest.mark.parametrize("k", [True, 4])
+def test_parafac_orthogonalise_matches_reference(cuda, k):
+    """orthogonalise=True QRs the factors before sweeps 0 and 1 (True is an
+    int), an int k before sweeps 0..k: the drop-in follows the reference's
+    recorded run (tests/golden/more_trajectories.npz)."""
+    import os
+    import tensorly_amd as tla
+    more = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "more_trajectories.npz"))
+    cp, errs = tla.parafac(g(more["ox"], cuda), 3, n_iter_max=8, init="random", random_state=5, tol=0,
+                           orthogonalise=k, return_errors=True)
+    np.testing.assert_allclose(errs, more[f"orth_{int(k)}_errors"], rtol=1e-9)
+    for m in range(3):
+        np.testing.assert_allclose(cp.factors[m].cpu().numpy(), more[f"orth_{int(k)}_factor{m}"], rtol=1e-7,
+                                   atol=1e-10)
+
+
+@pytest.mark.parametrize("init", ["random", "svd"])
+def test_parafac_mask_matches_reference(cuda, init):
+    """mask: the missing entries are re-imputed from the model after every
+    sweep and the error is that of the imputed tensor; init='svd' imputes the
+    unfoldings first (svd_mask_repeats=3).  Against the reference's recorded
+    runs (tests/golden/more_trajectories.npz)."""
+    import os
+    import tensorly_amd as tla
+    more = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "more_trajectories.npz"))
+    kw = dict(init="random", random_state=6) if init == "random" else dict(init="svd", svd_mask_repeats=3)
+    X = g(more["mx"], cuda)
+    cp, errs = tla.parafac(X, 3, n_iter_max=10, tol=0, mask=g(more["mmask"], cuda), return_errors=True, **kw)
+    np.testing.assert_allclose(errs, more[f"mask_{init}_errors"], rtol=1e-8)
+    for m in range(3):
+        np.testing.assert_allclose(cp.factors[m].cpu().numpy(), more[f"mask_{init}_factor{m}"], rtol=1e-6,
+                                   atol=1e-9)
+    # the caller's tensor is not modified
+    np.testing.assert_array_equal(X.cpu().numpy(), more["mx"])
+
+
+@pytest.mark.parametrize("linesearch", [False, True])
+def test_parafac_mask_float32_vs_oracle(cuda, linesearch):
+    """float32, rank 5, 30 % missing, with and without the line search,
+    against the FP64 oracle: err^2 within 1e-6 at every recorded sweep."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(60)
+    f = [rng.random((s, 5)) for s in (40, 36, 64)]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(5), f, dtype=np.float32), 0.01, rng)
+    mask = (rng.random(x.shape) > 0.3).astype(np.float32)
+    (_, _), ref = T.parafac(x.astype(np.float64), 5, n_iter_max=20, init="random", random_state=3, tol=0,
+                            mask=mask.astype(np.float64), linesearch=linesearch, return_errors=True)
+    _, errs = tla.parafac(g(x, cuda, torch.float32), 5, n_iter_max=20, init="random", random_state=3, tol=0,
+                          mask=g(mask, cuda, torch.float32), linesearch=linesearch, return_errors=True)
+    assert len(errs) == len(ref)
+    d = np.abs(np.array(errs) ** 2 - np.array(ref) ** 2)
+    assert np.max(d) < 1e-6, d
+
+
+def test_error_calc_with_mask(cuda):
+    import tensorly_amd as tla
+    from tensorly_amd.decomposition import error_calc
+    rng = np.random.default_rng(61)
+    x = rng.random((7, 8, 9))
+    mask = (rng.random(x.shape) > 0.4).astype(np.float64)
+    w = rng.random(3)
+    f = [rng.random((s, 3)) for s in x.shape]
+    e_ref, x_ref, n_ref = T.error_calc(x, np.linalg.norm(x), w, f, mask=mask)
+    e, xt, n = error_calc(g(x, cuda), float(np.linalg.norm(x)), g(w, cuda), [g(a, cuda) for a in f],
+                          mask=g(mask, cuda))
+    np.testing.assert_allclose(float(e), e_ref, rtol=1e-12)
+    np.testing.assert_allclose(float(n), n_ref, rtol=1e-12)
+    np.testing.assert_allclose(xt.cpu().numpy(), x_ref, rtol=1e-13)

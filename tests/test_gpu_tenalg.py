@@ -90,14 +90,16 @@ def test_mttkrp_float32(cuda, shape, rank, prec, tol):
 
 
 def test_mttkrp_signed_data_bf16x3(cuda):
-    """Cancellation-heavy signed data: bf16x3 stays at FP32-level error."""
+    """Cancellation-heavy signed data: bf16x3 within 1e-5 (measured 4.2-4.6e-6
+    on this case: the split operands carry 16 of FP32's 24 bits), plain BF16
+    within 1e-2."""
     import tensorly_amd as tla
     rng = np.random.default_rng(5)
     shape, rank = (32, 48, 64), 32
     X = rng.standard_normal(shape).astype(np.float32)
     f = [rng.standard_normal((s, rank)).astype(np.float32) for s in shape]
     Xg, fg = g(X, cuda, torch.float32), [g(x, cuda, torch.float32) for x in f]
-    for prec, tol in (("fp32", 1e-5), ("bf16x3", 3e-5), ("bf16", 3e-2)):
+    for prec, tol in (("fp32", 1e-5), ("bf16x3", 1e-5), ("bf16", 1e-2)):
         with tla.precision(prec):
             for mode in range(3):
                 ref = T.unfolding_dot_khatri_rao(X.astype(np.float64), (None, [x.astype(np.float64) for x in f]),
@@ -135,7 +137,7 @@ def test_last_mode_fj_staging(cuda, monkeypatch, fj, rank):
     f = [rng.standard_normal((s, rank)).astype(np.float32) for s in shape]
     Xg, fg = g(X, cuda, torch.float32), [g(x, cuda, torch.float32) for x in f]
     ref = T.unfolding_dot_khatri_rao(X.astype(np.float64), (None, [x.astype(np.float64) for x in f]), 2)
-    for prec, tol in (("fp32", 1e-5), ("bf16x3", 3e-5), ("bf16", 3e-2)):
+    for prec, tol in (("fp32", 1e-5), ("bf16x3", 1e-5), ("bf16", 1e-2)):
         with tla.precision(prec):
             monkeypatch.setenv("TLA_LAST_FJ", fj)
             out = tla.unfolding_dot_khatri_rao(Xg, (None, fg), 2)
@@ -235,7 +237,7 @@ def test_mode_dot_known_answers(cuda, known_answers):
 
 
 @pytest.mark.parametrize("shape,J", [((64, 96, 128), 32), ((32, 64, 256), 16), ((33, 20, 7), 5), ((16, 8, 32, 64), 48)])
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("bf16x3", 3e-5), ("bf16", 1e-2)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("bf16x3", 1e-5), ("bf16", 1e-2)])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_mode_dot_float32(cuda, shape, J, prec, tol, transpose):
     import tensorly_amd as tla
@@ -341,6 +343,6 @@ def test_operands_on_the_host_are_refused(cuda):
     coords = torch.stack([torch.arange(5, device=cuda) % s for s in (8, 9, 10)])
     vals = torch.ones(5, device=cuda)
     with pytest.raises(TypeError):
-        tla.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (None, host_f), 0)
+        tla.tenalg.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (None, host_f), 0)
     with pytest.raises(TypeError):
-        tla.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (w.cpu(), f), 0)
+        tla.tenalg.unfolding_dot_khatri_rao_coo(coords, vals, (8, 9, 10), (w.cpu(), f), 0)

@@ -162,3 +162,12 @@ def test_planted_prox_mistake_is_caught(more):
     x = more["prox_in_signed"]
     wrong = T.simplex_prox(x, 1.0)
     assert not np.allclose(wrong, more["prox_simplex_signed"], atol=1e-6)
+
+
+@pytest.mark.parametrize("init", ["random", "svd"])
+def test_parafac_mask_matches_reference(more, init):
+    kw = dict(init="random", random_state=6) if init == "random" else dict(init="svd", svd_mask_repeats=3)
+    (w, f), errs = T.parafac(more["mx"], 3, n_iter_max=10, tol=0, mask=more["mmask"], return_errors=True, **kw)
+    np.testing.assert_allclose(errs, more[f"mask_{init}_errors"], rtol=1e-9)
+    for m in range(3):
+        np.testing.assert_allclose(f[m], more[f"mask_{init}_factor{m}"], rtol=1e-7, atol=1e-10)
