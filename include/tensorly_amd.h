@@ -68,6 +68,23 @@ int tla_mttkrp(int x_dtype, const void* X, int ndim, const int64_t* shape,
                void* stream);
 
 /* ---------------------------------------------------------------------
+ * Allocation-free MTTKRP.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as tla_mttkrp, without the workspace: the scratch space is a fixed
+ * device array of tla_mttkrp_noalloc_max_scratch() bytes (64 MiB) that each
+ * GPU holds for the life of the process; an MTTKRP whose
+ * tla_mttkrp_workspace_size() exceeds it returns TLA_ERR_ARG.  The call
+ * allocates nothing.  Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (other streams or host threads) race.  The
+ * stream must belong to the current device.
+ * ------------------------------------------------------------------- */
+size_t tla_mttkrp_noalloc_max_scratch(void);
+int tla_mttkrp_noalloc(int x_dtype, const void* X, int ndim, const int64_t* shape,
+                       const void* const* factors, const void* weights, int rank, int mode,
+                       int precision, void* out, void* stream);
+
+/* ---------------------------------------------------------------------
  * Partial MTTKRP (dimension tree): contract every mode except the
  * contiguous block [keep_begin, keep_end) -- which must be a prefix
  * (keep_begin == 0) or a suffix (keep_end == ndim) of the modes -- with the

@@ -39,6 +39,8 @@ _SIGNATURES = {
     "tla_abi_version": (c_i, []),
     "tla_mttkrp_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i]),
     "tla_mttkrp": (c_i, [c_i, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_i, c_p, c_p, c_sz, c_p]),
+    "tla_mttkrp_noalloc_max_scratch": (c_sz, []),
+    "tla_mttkrp_noalloc": (c_i, [c_i, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_i, c_p, c_p]),
     "tla_mttkrp_partial_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i, c_i]),
     "tla_mttkrp_partial_supported": (c_i, [c_i, c_i, P_i64, c_i, c_i, c_i, c_i]),
     "tla_mttkrp_partial": (c_i, [c_i, c_p, c_i, P_i64, PP, c_p, c_i, c_i, c_i, c_i, c_p, c_p, c_sz, c_p]),
