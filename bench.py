@@ -33,6 +33,7 @@ Also reported on the same JSON line:
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 import argparse
+import gc
 import json
 import os
 import platform
@@ -66,8 +67,13 @@ def parse():
     p.add_argument("--quick", action="store_true", help="skip time-to-fit and the CPU baseline")
     p.add_argument("--fit-sweeps", type=int, default=300)
     p.add_argument("--no-batched", action="store_true", help="skip the batched workload")
-    p.add_argument("--kernel-only", choices=["partial01", "last", "mode0", "copy"],
-                   help="only launch one kernel --iters times on the 512^3 data (for counter runs)")
+    p.add_argument("--kernel-only", choices=["partial01", "last", "mode0", "copy", "c3", "batched", "ttm"],
+                   help="only launch the named kernel(s) --iters times, for counter runs: the 512^3 rank-32 "
+                        "MTTKRPs (partial01, last, mode0) or a 512 MiB copy; c3: the four 128^4 rank-64 "
+                        "MTTKRPs and the two dimension-tree partials; batched: one plain run of the "
+                        "4096-problem batch; ttm: the 256^3 rank-32 Tucker TTM chains")
+    p.add_argument("--kprec", choices=["fp32", "bf16", "bf16x3"], default="bf16x3",
+                   help="MTTKRP precision of --kernel-only runs")
     p.add_argument("--iters", type=int, default=5)
     p.add_argument("--model", default=None, help="extrapolation network file (default: the packaged one)")
     p.add_argument("--study", choices=["precision", "tucker"],
@@ -166,29 +172,68 @@ def time_to_fit(engine, n_sweeps, target_err, torch):
 
 
 def kernel_only(args, torch, dev):
-    """Launch one kernel --iters times on the 512^3 workload (bf16x3 for the
-    MTTKRP kernels) and nothing else, so per-dispatch hardware counters can be
-    read without attribution; 'copy' is a plain 512 MiB device copy used to
-    calibrate the counters' byte scale."""
+    """Launch one workload's kernels --iters times and nothing else, so
+    per-dispatch hardware counters can be read without attribution.  The
+    512^3 rank-32 MTTKRP kernels, or 'copy' (a plain 512 MiB device copy
+    that calibrates the counters' byte scale); 'c3': per iteration the
+    128^4 rank-64 MTTKRP of modes 0, 1, 2, 3, then the partials keeping
+    modes (0, 1) and (2, 3), in --kprec ('bf16' reads a bf16 copy of the
+    tensor); 'batched': one plain run (tol 1e-7, at most 1000 sweeps) of
+    the 4096-problem batch; 'ttm': per iteration the three two-factor TTM
+    chains of a 256^3 Tucker-HOOI iteration with rank-32 factors."""
     import tensorly_amd as tla
     from tensorly_amd.tenalg import mttkrp_partial
     torch.manual_seed(0)
-    X = torch.rand(SHAPE, device=dev)
-    f = [torch.rand((s, RANK), device=dev) for s in SHAPE]
-    if args.kernel_only == "copy":
-        Y = torch.empty_like(X)
-        fn = lambda: Y.copy_(X)  # noqa: E731
-    elif args.kernel_only == "partial01":
-        fn = lambda: mttkrp_partial(X, (None, f), (0, 2))  # noqa: E731
+    prec = args.kprec
+    if args.kernel_only == "c3":
+        X = torch.randn((128,) * 4, device=dev)
+        if prec == "bf16":
+            X = X.to(torch.bfloat16)
+        f = [torch.randn((128, 64), device=dev) for _ in range(4)]
+        f = [a / a.norm(dim=0, keepdim=True) for a in f]
+
+        def fn():
+            for mode in range(4):
+                tla.unfolding_dot_khatri_rao(X, (None, f), mode)
+            mttkrp_partial(X, (None, f), (0, 2))
+            mttkrp_partial(X, (None, f), (2, 4))
+    elif args.kernel_only == "batched":
+        from tensorly_amd import synthetic
+        from tensorly_amd.batched import BatchedCP
+        probs = synthetic.batched_cp(n_problems=4096, seed=4)
+        b = BatchedCP([x for x, _ in probs], [r for _, r in probs], device=dev)
+        f0 = b.random_init(seed=0)
+        out = b.make_outputs(1000)
+        fcur = f0.clone()
+
+        def fn():
+            fcur.copy_(f0)
+            b.launch(fcur, out, policy="plain", max_sweeps=1000, tol=1e-7)
+    elif args.kernel_only == "ttm":
+        X = torch.randn((256, 256, 256), device=dev)
+        U = [torch.linalg.qr(torch.randn((256, 32), device=dev))[0] for _ in range(3)]
+
+        def fn():
+            for m in range(3):
+                others = [k for k in range(3) if k != m]
+                tla.multi_mode_dot(X, [U[k] for k in others], modes=others, transpose=True)
     else:
-        mode = 2 if args.kernel_only == "last" else 0
-        fn = lambda: tla.unfolding_dot_khatri_rao(X, (None, f), mode)  # noqa: E731
+        X = torch.rand(SHAPE, device=dev)
+        f = [torch.rand((s, RANK), device=dev) for s in SHAPE]
+        if args.kernel_only == "copy":
+            Y = torch.empty_like(X)
+            fn = lambda: Y.copy_(X)  # noqa: E731
+        elif args.kernel_only == "partial01":
+            fn = lambda: mttkrp_partial(X, (None, f), (0, 2))  # noqa: E731
+        else:
+            mode = 2 if args.kernel_only == "last" else 0
+            fn = lambda: tla.unfolding_dot_khatri_rao(X, (None, f), mode)  # noqa: E731
     torch.cuda.synchronize()
-    with tla.precision("bf16x3"):
-        for _ in range(args.iters):
+    with tla.precision(prec):
+        for it in range(args.iters):
             fn()
-    torch.cuda.synchronize()
-    log(f"[bench] kernel-only {args.kernel_only}: {args.iters} launches")
+            torch.cuda.synchronize()
+            log(f"[bench] kernel-only {args.kernel_only} ({prec}): iteration {it + 1} of {args.iters}")
 
 
 def _pow2_column_scales(F):
@@ -340,12 +385,21 @@ def tucker_study(torch, dev):
     X = torch.from_numpy(x).to(dev)
     rank = [32, 32, 32]
     tla.tucker(X, rank, n_iter_max=2, init="svd", tol=0)  # warm-up
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    (core, factors), errs = tla.tucker(X, rank, n_iter_max=20, init="svd", tol=0, return_errors=True)
-    torch.cuda.synchronize()
-    gpu_s = time.perf_counter() - t1
-    log(f"[study] GPU tucker 20 iterations: {gpu_s:.4f}s")
+    # three timed calls, Python's cyclic garbage collector paused during
+    # each (as timeit does): a full collection in a process holding torch's
+    # objects takes tens of ms and otherwise lands inside a timed call
+    times = []
+    for _ in range(3):
+        gc.collect()
+        gc.disable()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        (core, factors), errs = tla.tucker(X, rank, n_iter_max=20, init="svd", tol=0, return_errors=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        gc.enable()
+    gpu_s = float(np.median(times))
+    log(f"[study] GPU tucker 20 iterations: {gpu_s:.4f}s (median of {[round(t, 4) for t in times]})")
     from threadpoolctl import threadpool_limits
     cores = os.cpu_count() or 1
     with threadpool_limits(cores):
@@ -364,7 +418,7 @@ def tucker_study(torch, dev):
     ttm_flops_iter = 3 * (2 * 256 * 32 * 256 * 256 + 2 * 256 * 32 * 32 * 256) + 2 * 256 * 32 * 32 * 32
     return {"workload": "256^3 float32 Tucker model, core 32^3 N(0,1), orthonormal factors, 1% noise; "
                         "tucker(rank=[32,32,32], n_iter_max=20, init='svd', tol=0)",
-            "gpu_seconds": gpu_s, "gpu_ms_per_iteration_incl_init": gpu_s / 20 * 1e3,
+            "gpu_seconds": gpu_s, "gpu_seconds_runs": times, "gpu_ms_per_iteration_incl_init": gpu_s / 20 * 1e3,
             "ttm_gflop_per_iteration": ttm_flops_iter / 1e9,
             "final_rec_error": float(errs[-1]), "oracle_final_rec_error": float(errs_r[-1]),
             "core_rel_diff": float(np.linalg.norm(core_g - core_r) / np.linalg.norm(core_r)),

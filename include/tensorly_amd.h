@@ -366,6 +366,14 @@ size_t tla_sym_eig_workspace_size(int n);
 int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals,
                 double* evecs, double tol, int max_iter, int* iters_out, void* workspace,
                 size_t workspace_bytes, void* stream);
+/* count (<= 8) independent tla_sym_eig problems in one launch per solver
+ * kind, one workgroup each (the three SVDs of the Tucker init,
+ * tensorly/decomposition/_tucker.py:63-79, run side by side).  Per-problem
+ * arrays; Q0 and k0 may be NULL (no warm starts), entries of Q0 NULL. */
+int tla_sym_eig_batch(int count, const double* const* G, const int* n, const int* k,
+                      const double* const* Q0, const int* k0, double* const* evals,
+                      double* const* evecs, double tol, int max_iter, void* const* workspace,
+                      const size_t* workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Native CP-ALS sweep driver: the mode loop of

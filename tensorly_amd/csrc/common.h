@@ -90,8 +90,44 @@ __global__ void __launch_bounds__(256) strided_reduce_kernel(const float* __rest
 
 inline void strided_reduce(const float* in, int64_t groups, int64_t E, int64_t K, int64_t gs, int64_t ks, float* out,
                            hipStream_t st) {
-  dim3 grid((unsigned)ceil_div(E, 64), (unsigned)groups);
-  strided_reduce_kernel<<<grid, 256, 0, st>>>(in, E, K, gs, ks, out);
+  for (int64_t g0 = 0; g0 < groups; g0 += 65535) {  // grid.y holds at most 65535 groups
+    const int64_t ng = groups - g0 < 65535 ? groups - g0 : 65535;
+    dim3 grid((unsigned)ceil_div(E, 64), (unsigned)ng);
+    strided_reduce_kernel<<<grid, 256, 0, st>>>(in + g0 * gs, E, K, gs, ks, out + g0 * E);
+  }
+}
+
+// Chunks C (a power of two dividing K, with K / C >= 16 terms each) that
+// strided_reduce_wide splits a sum of K terms into, so that its first pass
+// has about 1024 workgroups.  1 when the groups alone give that many, or
+// when the groups do not follow one another in memory (gs != K ks).
+inline int64_t reduce_chunks(int64_t groups, int64_t E, int64_t K, int64_t gs, int64_t ks) {
+  if (groups > 1 && gs != K * ks) return 1;
+  const int64_t blocks = groups * ((E + 63) / 64);
+  int64_t C = 1;
+  while (blocks * C < 1024 && K % (2 * C) == 0 && K / (2 * C) >= 16) C *= 2;
+  return C;
+}
+
+// floats of temporary storage strided_reduce_wide needs
+inline size_t reduce_tmp_elems(int64_t groups, int64_t E, int64_t K, int64_t gs, int64_t ks) {
+  const int64_t C = reduce_chunks(groups, E, K, gs, ks);
+  return C > 1 ? size_t(groups * C * E) : 0;
+}
+
+// strided_reduce in two passes when the groups alone are too few to fill
+// the GPU: partial sums over C chunks of the K terms into tmp, then the sum
+// of the C chunk sums.  The order of the additions is fixed.
+inline void strided_reduce_wide(const float* in, int64_t groups, int64_t E, int64_t K, int64_t gs, int64_t ks,
+                                float* out, float* tmp, hipStream_t st) {
+  const int64_t C = reduce_chunks(groups, E, K, gs, ks);
+  if (C == 1) {
+    strided_reduce(in, groups, E, K, gs, ks, out, st);
+    return;
+  }
+  const int64_t Kc = K / C;
+  strided_reduce(in, groups * C, E, Kc, Kc * ks, ks, tmp, st);
+  strided_reduce(tmp, groups, E, C, C * E, E, out, st);
 }
 
 // tla_cp_solve with the driver's options: the relative error of the

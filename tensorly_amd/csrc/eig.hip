@@ -271,6 +271,12 @@ struct EigArgs {
   double* T;  // orthonormalising factor when it does not fit LDS
 };
 
+// Problems solved by one launch, one workgroup each (blockIdx.x selects).
+constexpr int kEigMaxBatch = 8;
+struct EigBatch {
+  EigArgs a[kEigMaxBatch];
+};
+
 __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
   uint32_t h = i * 0x9E3779B1u ^ (j + 0x7F4A7C15u) * 0x85EBCA77u;
   h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
@@ -280,7 +286,8 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
 // Jacobi on the whole matrix: CAP 64 keeps A and W in LDS; CAP 128 keeps A
 // in LDS and W in the workspace (a.W, ld 128).
 template <int CAP>
-__global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigArgs a) {
+__global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch batch) {
+  const EigArgs a = batch.a[blockIdx.x];
   constexpr int LDA = CAP + 1;
   __shared__ double A[CAP * LDA];
   __shared__ double Wl[CAP == kJac ? CAP * LDA : 1];
@@ -376,7 +383,8 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
 // block: 48 and 64 keep H, W and T in LDS; 128 keeps H in LDS and W, T in the
 // workspace.  All products are FP64 MFMA on zero-padded np-row blocks.
 template <int B>
-__global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a) {
+__global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch batch) {
+  const EigArgs a = batch.a[blockIdx.x];
   constexpr int LDH = B + 1, CAP = B;
   constexpr bool SMALL = B <= 64;
   __shared__ double H[B * LDH];  // Ritz matrix; also S of the orthonormalisation; then sorted rotations
@@ -411,9 +419,18 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigArgs a
     Y[i * CAP + j] = i >= n ? 0.0 : ((a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j));
   }
   __syncthreads();
-  block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
-  block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
-  { double* t = V; V = Y; Y = t; }
+  if (a.Q0 && a.k0 > 0) {
+    // orthonormal warm-start columns and B - k0 pseudo-random ones: the
+    // block is well conditioned (cond(S) ~ 1e2), one unshifted Cholesky-QR
+    // pass leaves V orthonormal to ~1e-13
+    block_orth<B>(Y, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+  } else {
+    block_orth<B>(Y, V, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+    block_orth<B>(V, Y, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    double* t = V;
+    V = Y;
+    Y = t;
+  }
   const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
   double jthr = 1e-10;
   int it = 0;
@@ -608,28 +625,30 @@ extern "C" size_t tla_sym_eig_workspace_size(int n) {
   return align256(np * np * 8) + 4 * align256(np * kEigCapLarge * 8) + 2 * rot;
 }
 
-extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs,
-                           double tol, int max_iter, int* iters_out, void* ws, size_t ws_bytes, void* stream) {
+// Arguments of one problem (workspace carved as the kernel kind needs);
+// *kind: 0 Jacobi in LDS, 1 Jacobi with rotations in the workspace, 2/3/4
+// subspace iteration with block 48 / 64 / 128.
+static int eig_args(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs, double tol,
+                    int max_iter, int* iters_out, void* ws, size_t ws_bytes, EigArgs* out, int* kind) {
   TLA_REQUIRE(n >= 1, "sym_eig: n = %d", n);
   TLA_REQUIRE(k >= 1 && k <= n, "sym_eig: k = %d not in [1, n = %d]", k, n);
   TLA_REQUIRE(G && evals && evecs, "sym_eig: NULL pointer");
   TLA_REQUIRE(!Q0 || k0 >= 1, "sym_eig: warm start with k0 = %d columns", k0);
   TLA_REQUIRE(ws_bytes >= tla_sym_eig_workspace_size(n), "sym_eig: workspace too small");
   TLA_REQUIRE(n <= kJacLarge || ws, "sym_eig: NULL workspace");
-  hipStream_t st = (hipStream_t)stream;
   EigArgs a{};
   a.G = G; a.n = n; a.k = k; a.Q0 = Q0; a.q0_ld = k0;
   a.evals = evals; a.evecs = evecs; a.tol = tol > 0 ? tol : 1e-11; a.max_iter = max_iter > 0 ? max_iter : 200;
   a.iters = iters_out;
   if (n <= kJac) {
-    sym_eig_small_kernel<kJac><<<1, kEigThreads, 0, st>>>(a);
-    TLA_LAUNCH_CHECK();
+    *out = a;
+    *kind = 0;
     return TLA_OK;
   }
   if (n <= kJacLarge) {
     a.W = (double*)ws;
-    sym_eig_small_kernel<kJacLarge><<<1, kEigThreads, 0, st>>>(a);
-    TLA_LAUNCH_CHECK();
+    *out = a;
+    *kind = 1;
     return TLA_OK;
   }
   // block: 48 for k <= 40 (cheaper Jacobi sweeps), 64 for k <= 56, 128 for k <= 120
@@ -654,12 +673,52 @@ extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int 
   a.W = (double*)(base + 4 * blk);
   a.T = (double*)(base + 4 * blk + rot);
   a.k0 = Q0 ? std::min(k0, b) : 0;
-  if (b == 48)
-    sym_eig_subspace_kernel<48><<<1, kEigThreads, 0, st>>>(a);
-  else if (small_block)
-    sym_eig_subspace_kernel<64><<<1, kEigThreads, 0, st>>>(a);
-  else
-    sym_eig_subspace_kernel<kEigCapLarge><<<1, kEigThreads, 0, st>>>(a);
-  TLA_LAUNCH_CHECK();
+  *out = a;
+  *kind = b == 48 ? 2 : (b == 64 ? 3 : 4);
   return TLA_OK;
+}
+
+// One launch per kernel kind present, one workgroup per problem.
+static int eig_launch(const EigArgs* args, const int* kinds, int count, hipStream_t st) {
+  for (int kind = 0; kind < 5; ++kind) {
+    EigBatch b{};
+    int m = 0;
+    for (int i = 0; i < count; ++i)
+      if (kinds[i] == kind) b.a[m++] = args[i];
+    if (m == 0) continue;
+    switch (kind) {
+      case 0: sym_eig_small_kernel<kJac><<<m, kEigThreads, 0, st>>>(b); break;
+      case 1: sym_eig_small_kernel<kJacLarge><<<m, kEigThreads, 0, st>>>(b); break;
+      case 2: sym_eig_subspace_kernel<48><<<m, kEigThreads, 0, st>>>(b); break;
+      case 3: sym_eig_subspace_kernel<64><<<m, kEigThreads, 0, st>>>(b); break;
+      default: sym_eig_subspace_kernel<kEigCapLarge><<<m, kEigThreads, 0, st>>>(b); break;
+    }
+    TLA_LAUNCH_CHECK();
+  }
+  return TLA_OK;
+}
+
+extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs,
+                           double tol, int max_iter, int* iters_out, void* ws, size_t ws_bytes, void* stream) {
+  EigArgs a;
+  int kind = 0;
+  const int rc = eig_args(G, n, k, Q0, k0, evals, evecs, tol, max_iter, iters_out, ws, ws_bytes, &a, &kind);
+  if (rc != TLA_OK) return rc;
+  return eig_launch(&a, &kind, 1, (hipStream_t)stream);
+}
+
+extern "C" int tla_sym_eig_batch(int count, const double* const* G, const int* n, const int* k,
+                                 const double* const* Q0, const int* k0, double* const* evals, double* const* evecs,
+                                 double tol, int max_iter, void* const* workspace, const size_t* workspace_bytes,
+                                 void* stream) {
+  TLA_REQUIRE(count >= 1 && count <= kEigMaxBatch, "sym_eig_batch: count %d not in [1, %d]", count, kEigMaxBatch);
+  TLA_REQUIRE(G && n && k && evals && evecs && workspace && workspace_bytes, "sym_eig_batch: NULL array");
+  EigArgs args[kEigMaxBatch];
+  int kinds[kEigMaxBatch];
+  for (int i = 0; i < count; ++i) {
+    const int rc = eig_args(G[i], n[i], k[i], Q0 ? Q0[i] : nullptr, k0 ? k0[i] : 0, evals[i], evecs[i], tol,
+                            max_iter, nullptr, workspace[i], workspace_bytes[i], &args[i], &kinds[i]);
+    if (rc != TLA_OK) return rc;
+  }
+  return eig_launch(args, kinds, count, (hipStream_t)stream);
 }

@@ -644,8 +644,13 @@ __device__ __forceinline__ void stream_finish_tile(const StreamArgs& args, int64
   }
 }
 
+// Chunks in flight per wave (measured: 5 gains nothing over 3 at K = 128
+// and K = 512).
+constexpr int kStreamDepth = 3;
+
 template <int PREC, typename XT, int RT, int EPI>
 __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(StreamArgs args) {
+  constexpr int DEPTH = kStreamDepth;
   constexpr int RP = RT * 16;
   constexpr int NT = 64 * kStreamWaves;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -688,7 +693,8 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   const int cpt = K / 64;  // 64-column chunks per tile
   const int64_t nch = my_tiles * cpt;
   // chunk i covers tile w + (i / cpt) * W, columns (i % cpt) * 64 .. +63.
-  // Three-deep ring: chunk i + 2 is requested while chunk i is multiplied.
+  // DEPTH-deep ring: chunk i + DEPTH - 1 is requested while chunk i is
+  // multiplied.
   int64_t ld_tile = w;
   int ld_kk = 0;
   auto issue = [&](StreamSlot<RT>& sl) {
@@ -716,9 +722,10 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
     ld_kk += 64;
     if (ld_kk == K) { ld_kk = 0; ld_tile += W; }
   };
-  StreamSlot<RT> s0, s1, s2;
-  if (nch > 0) issue(s0);
-  if (nch > 1) issue(s1);
+  StreamSlot<RT> ring[DEPTH];
+#pragma unroll
+  for (int s = 0; s < DEPTH - 1; ++s)
+    if (s < nch) issue(ring[s]);
   int64_t cur_tile = w;
   int cur_kk = 0;
   floatx4 d[RT];
@@ -736,15 +743,13 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
       cur_tile += W;
     }
   };
-  for (int64_t i = 0; i < nch; i += 3) {
-    if (i + 2 < nch) issue(s2);
-    consume(s0);
-    if (i + 1 >= nch) break;
-    if (i + 3 < nch) issue(s0);
-    consume(s1);
-    if (i + 2 >= nch) break;
-    if (i + 4 < nch) issue(s1);
-    consume(s2);
+  for (int64_t i = 0; i < nch; i += DEPTH) {
+#pragma unroll
+    for (int s = 0; s < DEPTH; ++s) {
+      if (i + s >= nch) break;
+      if (i + s + DEPTH - 1 < nch) issue(ring[(s + DEPTH - 1) % DEPTH]);
+      consume(ring[s]);
+    }
   }
   if constexpr (EPI == SE_ACCJ) {
     // this wave's output rows: 16 * (w % tpq) + 4 g + e
@@ -771,20 +776,24 @@ __global__ void accj_reduce_kernel(const float* __restrict__ partial, int64_t W,
   out[e] = acc;
 }
 
-// M[i, r] = sum_{a, b} sum_{s < S} partial[((a * In + i) * Pb + b) * S + s][r]
-__global__ void tiles_reduce_kernel(const float* __restrict__ partial, int64_t Pa, int64_t In, int64_t Pb, int S,
-                                    int R, float* __restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= In * R) return;
-  const int r = (int)(e % R);
-  const int64_t i = e / R;
-  float acc = 0.0f;
-  for (int64_t a = 0; a < Pa; ++a)
-    for (int64_t b = 0; b < Pb; ++b) {
-      const int64_t u0 = ((a * In + i) * Pb + b) * S;
-      for (int s = 0; s < S; ++s) acc += partial[(u0 + s) * R + r];
-    }
-  out[e] = acc;
+// M[i, r] = sum_{a < Pa} sum_{u < U} partial[((a In + i) U + u) R + r]
+// (u = (b, s): the tiles of one output row i for one a): first over a into
+// tmp (In U R floats) when Pa > 1, then over u.  Both passes are wide
+// strided reductions.
+static size_t tiles_reduce_tmp_elems(int64_t Pa, int64_t In, int64_t U, int R) {
+  return (Pa > 1 ? size_t(In * U * R) : 0) + reduce_tmp_elems(In, R, U, U * R, R);
+}
+
+static void tiles_reduce(const float* partial, int64_t Pa, int64_t In, int64_t U, int R, float* tmp, float* out,
+                         hipStream_t st) {
+  const float* src = partial;
+  float* tmp2 = tmp;
+  if (Pa > 1) {
+    strided_reduce(partial, In, U * R, Pa, U * R, In * U * R, tmp, st);
+    src = tmp;
+    tmp2 = tmp + In * U * R;
+  }
+  strided_reduce_wide(src, In, R, U, U * R, R, out, tmp2, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1221,6 +1230,7 @@ struct Plan {
   int64_t out_rows;
   int splits;
   size_t partial_elems;
+  size_t red_off;  // workspace offset (bytes) of the reductions' temporary storage
   size_t ws_bytes;
 };
 
@@ -1346,6 +1356,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
         p->gm_rows = (p->sepi == SE_ROWS) ? 0 : s.ntiles / tpq;
         sq.rows = p->gm_rows;
         p->ws_bytes = align256(size_t(p->gm_rows) * R * 4) + align256(p->partial_elems * 4);
+        p->red_off = p->ws_bytes;
+        if (p->sepi == SE_REDQ) p->ws_bytes += align256(tiles_reduce_tmp_elems(Pa, In, p->t_Pb * tpq, R) * 4);
         return TLA_OK;
       }
     }
@@ -1381,6 +1393,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
                                              : size_t(2) * RT * 16 * inner_pitch_bf16(t.panel) * 2;
       p->partial_elems = rows_epi ? 0 : size_t(t.n_units) * R;
       p->ws_bytes = align256(p->partial_elems * 4);
+      p->red_off = p->ws_bytes;
+      if (!rows_epi) p->ws_bytes += align256(tiles_reduce_tmp_elems(Pa, In, Pbr * p->t_S, R) * 4);
       return TLA_OK;
     }
     // more K panels than a grid dimension holds: the generic kernel
@@ -1410,6 +1424,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     p->smem = std::max(stage, size_t(4 * 16 * RT * 16 * 4));
     p->partial_elems = size_t(p->splits) * In * R;
     p->ws_bytes = align256(p->partial_elems * 4);
+    p->red_off = p->ws_bytes;
+    p->ws_bytes += align256(reduce_tmp_elems(1, In * R, p->splits, 0, In * R) * 4);
     return TLA_OK;
   }
   // Output block is a suffix including the contiguous mode: last kernel.
@@ -1463,6 +1479,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
     p->splits = (int)(G * s.rsplit);
     p->partial_elems = size_t(p->splits) * Kc * R;
     p->ws_bytes = align256(size_t(gm_rows) * R * 4) + align256(p->partial_elems * 4);
+    p->red_off = p->ws_bytes;
+    p->ws_bytes += align256(reduce_tmp_elems(1, Kc * R, p->splits, 0, Kc * R) * 4);
     return TLA_OK;
   }
   p->path = Path::Last;
@@ -1484,6 +1502,8 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
   p->splits = (int)S;
   p->partial_elems = size_t(S) * Kc * R;
   p->ws_bytes = align256(size_t(gm_rows) * R * 4) + align256(p->partial_elems * 4);
+  p->red_off = p->ws_bytes;
+  p->ws_bytes += align256(reduce_tmp_elems(1, Kc * R, S, 0, Kc * R) * 4);
   return TLA_OK;
 }
 
@@ -1707,11 +1727,7 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     else rc = launch_stream<TLA_PREC_BF16X3, float>(p, st);
     if (rc != TLA_OK || rows_out) return rc;
     if (p.sepi == SE_REDQ) {
-      const int64_t n = p.t_In * R;
-      if (p.t_Pa == 1)
-        strided_reduce(part, p.t_In, R, p.t_Pb * p.t_S, p.t_Pb * p.t_S * R, R, (float*)out, st);
-      else
-        tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>(part, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R, (float*)out);
+      tiles_reduce(part, p.t_Pa, p.t_In, p.t_Pb * p.t_S, R, (float*)((char*)ws + p.red_off), (float*)out, st);
     } else {
       const int64_t n = s.J * R;
       (void)n;
@@ -1745,7 +1761,7 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     else rc = launch_last_stream<TLA_PREC_BF16X3, float>(p, st);
     if (rc != TLA_OK) return rc;
     const int64_t n = p.out_rows * R;
-    strided_reduce(partial, 1, n, p.splits, 0, n, (float*)out, st);
+    strided_reduce_wide(partial, 1, n, p.splits, 0, n, (float*)out, (float*)((char*)ws + p.red_off), st);
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
@@ -1763,9 +1779,8 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     else if (precision == TLA_PREC_BF16) rc = launch_tiles<TLA_PREC_BF16, float>(p, st);
     else rc = launch_tiles<TLA_PREC_BF16X3, float>(p, st);
     if (rc != TLA_OK || p.epi == EPI_ROWS) return rc;
-    const int64_t n = p.t_In * R;
-    tiles_reduce_kernel<<<ceil_div(n, 256), 256, 0, st>>>((const float*)ws, p.t_Pa, p.t_In, p.t_Pb, p.t_S, R,
-                                                          (float*)out);
+    tiles_reduce((const float*)ws, p.t_Pa, p.t_In, p.t_Pb * p.t_S, R, (float*)((char*)ws + p.red_off), (float*)out,
+                 st);
     TLA_LAUNCH_CHECK();
     return TLA_OK;
   }
@@ -1802,7 +1817,7 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
   }
   if (rc != TLA_OK) return rc;
   const int64_t n = p.out_rows * R;
-  strided_reduce(partial, 1, n, p.splits, 0, n, (float*)out, st);
+  strided_reduce_wide(partial, 1, n, p.splits, 0, n, (float*)out, (float*)((char*)ws + p.red_off), st);
   TLA_LAUNCH_CHECK();
   return TLA_OK;
 }

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..svd import leading_left_singular_vectors
+from ..svd import leading_left_singular_vectors, leading_left_singular_vectors_batch
 from ..tenalg import multi_mode_dot, precision as precision_ctx, get_precision
 from ..tucker_tensor import TuckerTensor, validate_tucker_rank
 
@@ -36,14 +36,20 @@ def _sqnorm64(x):
 
 
 def initialize_tucker(tensor, rank, modes, random_state, init="svd", svd="truncated_svd", non_negative=False,
-                      mask=None, svd_mask_repeats=5):
-    """``_tucker.py:25-102`` (no mask)."""
+                      mask=None, svd_mask_repeats=5, warm64=None):
+    """``_tucker.py:25-102`` (no mask).  With init='svd' the modes' SVDs are
+    independent: their Gram eigensolves run side by side in one launch.
+    ``warm64`` (a list, optional) receives the FP64 factors of the SVD
+    init."""
     if mask is not None:
         raise NotImplementedError("mask is not supported by tensorly_amd.tucker")
     if isinstance(init, str) and init == "svd":
         if svd not in ("truncated_svd", "symeig_svd"):
             raise ValueError(f"Got svd={svd}. However, the possible choices are ['truncated_svd', 'symeig_svd'].")
-        factors = [leading_left_singular_vectors(tensor, mode, rank[i])[0] for i, mode in enumerate(modes)]
+        res = leading_left_singular_vectors_batch(tensor, list(modes), [rank[i] for i in range(len(modes))])
+        factors = [r[0] for r in res]
+        if warm64 is not None:
+            warm64.extend(r[2] for r in res)
         core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
     elif isinstance(init, str) and init == "random":
         from ._cp import _check_random_state
@@ -80,7 +86,14 @@ def partial_tucker(tensor, rank, modes=None, n_iter_max=100, init="svd", tol=10e
     else:
         rank = tuple(rank)
     with precision_ctx(precision or get_precision()):
-        core, factors = initialize_tucker(tensor, rank, modes, init=init, svd=svd, random_state=random_state)
+        # FP64 copies of the factors warm-start the next eigensolve of their
+        # mode: accurate past float32, a converged HOOI needs one subspace
+        # step per solve
+        warm64 = []
+        core, factors = initialize_tucker(tensor, rank, modes, init=init, svd=svd, random_state=random_state,
+                                          warm64=warm64)
+        if len(warm64) != len(modes):
+            warm64 = [None] * len(modes)
         rec_errors = []
         nx2 = _sqnorm64(tensor)
         norm_tensor = torch.sqrt(nx2)
@@ -88,7 +101,9 @@ def partial_tucker(tensor, rank, modes=None, n_iter_max=100, init="svd", tol=10e
         for iteration in range(n_iter_max):
             for index, mode in enumerate(modes):
                 y = multi_mode_dot(tensor, factors, modes=modes, skip=index, transpose=True)
-                U, _ = leading_left_singular_vectors(y, mode, rank[index], warm=factors[index])
+                warm = warm64[index] if warm64[index] is not None else factors[index]
+                U, _, warm64[index] = leading_left_singular_vectors(y, mode, rank[index], warm=warm,
+                                                                    return_fp64=True)
                 factors[index] = U
             core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
             err = torch.sqrt(torch.abs(nx2 - _sqnorm64(core))) / norm_tensor

@@ -50,6 +50,37 @@ def gram_eigh(G, k, warm=None, tol=1e-11, max_iter=300):
     return evals, evecs
 
 
+def gram_eigh_batch(Gs, ks, warms=None, tol=1e-11, max_iter=300):
+    """:func:`gram_eigh` of independent Grams side by side: one launch per
+    solver kind, one workgroup per Gram (``tla_sym_eig_batch``).  Returns a
+    list of ``(evals, evecs)``."""
+    out = []
+    for c0 in range(0, len(Gs), 8):
+        chunk = list(range(c0, min(len(Gs), c0 + 8)))
+        dev = Gs[chunk[0]].device
+        G = [Gs[i].to(torch.float64).contiguous() for i in chunk]
+        n = [int(g.shape[0]) for g in G]
+        k = [int(ks[i]) for i in chunk]
+        q0 = [warms[i].to(torch.float64).contiguous() if warms is not None and warms[i] is not None else None
+              for i in chunk]
+        evals = [torch.empty(kk, dtype=torch.float64, device=dev) for kk in k]
+        evecs = [torch.empty((nn, kk), dtype=torch.float64, device=dev) for nn, kk in zip(n, k)]
+        nbs = [int(lib().tla_sym_eig_workspace_size(nn)) for nn in n]
+        wss = [workspace(nb, dev) for nb in nbs]
+        Ip = _lib.ctypes.c_int * len(chunk)
+        gp, _kg = _lib.ptr_array(G)
+        qp, _kq = _lib.ptr_array(q0)
+        ep, _ke = _lib.ptr_array(evals)
+        vp, _kv = _lib.ptr_array(evecs)
+        wp, _kw = _lib.ptr_array(wss)
+        nbarr = (_lib.ctypes.c_size_t * len(chunk))(*nbs)
+        k0 = Ip(*[q.shape[1] if q is not None else 0 for q in q0])
+        check(lib().tla_sym_eig_batch(len(chunk), gp, Ip(*n), Ip(*k), qp, k0, ep, vp, float(tol), int(max_iter),
+                                      wp, nbarr, stream(dev)), "sym_eig_batch")
+        out.extend(zip(evals, evecs))
+    return out
+
+
 def svd_flip(U, V, u_based_decision=True):
     """Sign rule of ``svd.py:13-65`` (on GPU tensors)."""
     if u_based_decision:
@@ -177,16 +208,38 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
     return U, S, V
 
 
-def leading_left_singular_vectors(tensor, mode, k, warm=None):
+def leading_left_singular_vectors(tensor, mode, k, warm=None, return_fp64=False):
     """Leading ``k`` left singular vectors of ``unfold(tensor, mode)`` with the
     reference sign rule, and the singular values -- without forming the
-    unfolding (the HOOI / init path)."""
+    unfolding (the HOOI / init path).  ``return_fp64``: also the FP64
+    vectors (a warm start accurate beyond the tensor's precision)."""
     n = tensor.shape[mode]
     q = tensor.numel() // max(n, 1)
     G = mode_gram(tensor, mode)
     U, S = _left_svd_from_gram(G, n, min(n, q), k, warm=warm)
     U, _ = svd_flip(U, None)
+    if return_fp64:
+        return U.to(tensor.dtype), S.to(tensor.dtype), U
     return U.to(tensor.dtype), S.to(tensor.dtype)
+
+
+def leading_left_singular_vectors_batch(tensor, modes, ks):
+    """:func:`leading_left_singular_vectors` (with ``return_fp64``) of several
+    modes of one tensor, cold-started, the eigensolves in one batched launch
+    (the Tucker SVD init): a list of ``(U, S, U64)``."""
+    Gs, kk, dims = [], [], []
+    for mode, k in zip(modes, ks):
+        n = tensor.shape[mode]
+        q = tensor.numel() // max(n, 1)
+        Gs.append(mode_gram(tensor, mode))
+        kk.append(min(k, n))
+        dims.append(min(n, q))
+    res = []
+    for (evals, evecs), k, md in zip(gram_eigh_batch(Gs, kk), ks, dims):
+        s = torch.sqrt(torch.clamp(evals, min=0.0))[: min(k, md)]
+        U, _ = svd_flip(evecs, None)
+        res.append((U.to(tensor.dtype), s.to(tensor.dtype), U))
+    return res
 
 
 def qr(matrix):

