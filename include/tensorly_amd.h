@@ -41,7 +41,8 @@ extern "C" {
 
 /* arithmetic of the tensor contraction */
 #define TLA_PREC_FP32 0   /* FP32 MFMA, FP32 accumulation */
-#define TLA_PREC_BF16 1   /* BF16 MFMA, FP32 accumulation */
+#define TLA_PREC_BF16 1   /* BF16 MFMA: tensor rounded to bf16 (or a bf16 copy) times the
+                             factor / matrix operand split hi + lo, 2 MFMAs, FP32 accum. */
 #define TLA_PREC_BF16X3 2 /* split hi+lo BF16, 3 MFMAs per product, FP32 accum. */
 #define TLA_PREC_VALU 3   /* plain vector ALU in the tensor's own type */
 

@@ -23,9 +23,12 @@
 //
 //  * mttkrp_generic  plain vector-ALU fallback for any shape / rank / dtype.
 //
-// Precisions: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_16x16x32_bf16
-// on RNE-rounded operands) and BF16X3 (x = hi + lo split of both operands,
-// hi*hi + hi*lo + lo*hi, FP32 accumulation).  Partial sums of the split
+// Precisions: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_16x16x32_bf16:
+// the tensor RNE-rounded to bf16, or a bf16 copy of it, times the factor
+// side split as hi + lo, two products: exact for the rounded tensor up to
+// FP32 accumulation, so ALS stays consistent with its own Grams) and BF16X3
+// (x = hi + lo split of both operands, hi*hi + hi*lo + lo*hi, FP32
+// accumulation).  Partial sums of the split
 // reduction are summed in a fixed order by mttkrp_reduce (deterministic).
 #include <stdarg.h>
 #include <stdio.h>
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(256) void mttkrp_inner_kernel(InnerArgs args) {
     } else {
       const __bf16 h = (__bf16)v;
       Ch[n * LDB + kk] = h;
-      if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+      Cl[n * LDB + kk] = (__bf16)(v - (float)h);
     }
   }
   __syncthreads();
@@ -244,12 +247,10 @@ __global__ __launch_bounds__(256) void mttkrp_inner_kernel(InnerArgs args) {
         for (int nb = 0; nb < RT; ++nb) {
           const int off = (nb * 16 + c) * LDB + kk + 8 * g;
           const bf16x8 bh = *(const bf16x8*)(Ch + off);
+          const bf16x8 bl = *(const bf16x8*)(Cl + off);
           d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
-          if constexpr (PREC == TLA_PREC_BF16X3) {
-            const bf16x8 bl = *(const bf16x8*)(Cl + off);
-            d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
-            d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
-          }
+          d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
         }
       }
     }
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(256) void mttkrp_tiles_kernel(TilesArgs args) {
       } else {
         const __bf16 h = (__bf16)v;
         Ch[n * LDB + kk] = h;
-        if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+        Cl[n * LDB + kk] = (__bf16)(v - (float)h);
       }
     }
     __syncthreads();
@@ -386,16 +387,13 @@ __global__ __launch_bounds__(256) void mttkrp_tiles_kernel(TilesArgs args) {
           for (int nb = 0; nb < RT; ++nb) {
             const int off = (nb * 16 + c) * LDB + kk + 8 * g;
             const bf16x8 bh = *(const bf16x8*)(Ch + off);
-            bf16x8 bl;
-            if constexpr (PREC == TLA_PREC_BF16X3) bl = *(const bf16x8*)(Cl + off);
+            const bf16x8 bl = *(const bf16x8*)(Cl + off);
 #pragma unroll
             for (int t = 0; t < TPW; ++t) {
               if (t < tpu) {
                 d[t][nb] = mfma_bf16_16x16x32(ah[t], bh, d[t][nb]);
-                if constexpr (PREC == TLA_PREC_BF16X3) {
-                  d[t][nb] = mfma_bf16_16x16x32(ah[t], bl, d[t][nb]);
-                  d[t][nb] = mfma_bf16_16x16x32(al[t], bh, d[t][nb]);
-                }
+                d[t][nb] = mfma_bf16_16x16x32(ah[t], bl, d[t][nb]);
+                if constexpr (PREC == TLA_PREC_BF16X3) d[t][nb] = mfma_bf16_16x16x32(al[t], bh, d[t][nb]);
               }
             }
           }
@@ -487,10 +485,16 @@ __global__ __launch_bounds__(256) void mttkrp_tiles_kernel(TilesArgs args) {
 //   SE_ACCJ   (output mode = mode N-2, rows of a tile are output rows):
 //             acc[rows] += s_q * D, W % (J/16) == 0 keeps the wave on one
 //             16-row output block; partial[wave, 16, r].
+//   SE_REDQG  SE_REDQ with the J/16 tiles of one q visited by one wave in a
+//             row (wave w takes q = w, w + W, ...): the row sums accumulate
+//             in registers over the q and one partial per q is stored;
+//             F_j is staged in LDS (transposed) instead of riding in the
+//             prefetch slot.  Chosen when there are at least 2 W values of q.
 // ---------------------------------------------------------------------------
 constexpr int SE_ROWS = 0;
 constexpr int SE_REDQ = 1;
 constexpr int SE_ACCJ = 2;
+constexpr int SE_REDQG = 3;
 
 struct StreamArgs {
   const void* X;
@@ -589,12 +593,10 @@ __device__ __forceinline__ void stream_step(const floatx8& av, const float* Ct, 
     for (int nb = 0; nb < RT; ++nb) {
       const int off = (nb * 16 + c) * LDB + kk + 8 * g;
       const bf16x8 bh = *(const bf16x8*)(Ch + off);
+      const bf16x8 bl = *(const bf16x8*)(Cl + off);
       d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
-      if constexpr (PREC == TLA_PREC_BF16X3) {
-        const bf16x8 bl = *(const bf16x8*)(Cl + off);
-        d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
-        d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
-      }
+      d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+      if constexpr (PREC == TLA_PREC_BF16X3) d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
     }
   }
 }
@@ -615,9 +617,36 @@ struct StreamSlot {
 template <int PREC, typename XT, int RT, int EPI>
 __device__ __forceinline__ void stream_finish_tile(const StreamArgs& args, int64_t tile, int c, int g,
                                                    const StreamSlot<RT>& sl, const float wr[RT], floatx4 d[RT],
-                                                   floatx4 acc[RT]) {
+                                                   floatx4 acc[RT], const float* FjT, int LDJ) {
   const int R = args.R;
   const int64_t row0 = tile * 16;
+  if constexpr (EPI == SE_REDQG) {
+    // acc[nb][0]: this lane's part of sum_rows F_j[j, r] D[row, r] over the
+    // q's tiles so far (rows 4 g .. 4 g + 3 of each tile)
+    const int64_t tpq = args.J / 16;
+    const int64_t q = tile / tpq;
+    const int j0 = (int)(row0 - q * args.J);
+#pragma unroll
+    for (int nb = 0; nb < RT; ++nb) {
+      const floatx4 f = *(const floatx4*)(FjT + (nb * 16 + c) * LDJ + j0 + 4 * g);
+      float t = acc[nb][0];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) t = fmaf(f[e], d[nb][e], t);
+      acc[nb][0] = t;
+    }
+    if ((tile + 1) % tpq == 0) {
+#pragma unroll
+      for (int nb = 0; nb < RT; ++nb) {
+        const int r = nb * 16 + c;
+        float t = acc[nb][0];
+        t += __shfl_xor(t, 16, 64);
+        t += __shfl_xor(t, 32, 64);
+        if (g == 0 && r < R) args.out[q * R + r] = sl.sq[nb] * t;
+        acc[nb][0] = 0.0f;
+      }
+    }
+    return;
+  }
   // d[nb][e] = D[row0 + 4 g + e][nb * 16 + c]
   if constexpr (EPI == SE_ROWS) {
 #pragma unroll
@@ -671,7 +700,18 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
     } else {
       const __bf16 h = (__bf16)v;
       Ch[n * LDB + kk] = h;
-      if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+      Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+    }
+  }
+  // SE_REDQG: F_j transposed after the F_k panel, [RP][J + 4] (16-byte rows,
+  // zero past R)
+  const int LDJ = (int)args.J + 4;
+  float* FjT = (float*)(smem + (PREC == TLA_PREC_FP32 ? size_t(RP) * LDF * 4 : size_t(2) * RP * LDB * 2));
+  if constexpr (EPI == SE_REDQG) {
+    const int J = (int)args.J;
+    for (int idx = tid; idx < RP * J; idx += NT) {
+      const int n = idx % RP, j = idx / RP;
+      FjT[n * LDJ + j] = (n < R) ? args.Fj[(int64_t)j * R + n] : 0.0f;
     }
   }
   float wr[RT];
@@ -689,18 +729,36 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   floatx4 acc[RT];
 #pragma unroll
   for (int nb = 0; nb < RT; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int64_t my_tiles = (w < args.ntiles) ? (args.ntiles - w + W - 1) / W : 0;
+  // tiles this wave visits: w, w + W, ...; SE_REDQG: all tpq tiles of
+  // q = w, w + W, ... in turn
+  const int64_t nq = args.ntiles / tpq;
+  const int64_t my_tiles = (EPI == SE_REDQG) ? ((w < nq) ? (nq - w + W - 1) / W * tpq : 0)
+                                             : ((w < args.ntiles) ? (args.ntiles - w + W - 1) / W : 0);
   const int cpt = K / 64;  // 64-column chunks per tile
   const int64_t nch = my_tiles * cpt;
+  auto next_tile = [&](int64_t t) -> int64_t {
+    if constexpr (EPI == SE_REDQG) return ((t + 1) % tpq == 0) ? t + 1 + (W - 1) * tpq : t + 1;
+    else return t + W;
+  };
+  const int64_t first_tile = (EPI == SE_REDQG) ? w * tpq : w;
   // chunk i covers tile w + (i / cpt) * W, columns (i % cpt) * 64 .. +63.
   // DEPTH-deep ring: chunk i + DEPTH - 1 is requested while chunk i is
   // multiplied.
-  int64_t ld_tile = w;
+  int64_t ld_tile = first_tile;
   int ld_kk = 0;
   auto issue = [&](StreamSlot<RT>& sl) {
     load_pair<XT, EPI != SE_ROWS>(X + (ld_tile * 16 + c) * args.K + ld_kk + 8 * g, sl.a, sl.b);
     if (ld_kk + 64 == K) {
-      if constexpr (EPI != SE_ROWS) {
+      if constexpr (EPI == SE_REDQG) {
+        if ((ld_tile + 1) % tpq == 0) {  // the q's last tile stores its partial
+          const int64_t q = ld_tile / tpq;
+#pragma unroll
+          for (int nb = 0; nb < RT; ++nb) {
+            const int r = nb * 16 + c;
+            sl.sq[nb] = (r < R) ? args.Sq[q * R + r] : 0.0f;
+          }
+        }
+      } else if constexpr (EPI != SE_ROWS) {
         const int64_t q = ld_tile / tpq;
 #pragma unroll
         for (int nb = 0; nb < RT; ++nb) {
@@ -720,13 +778,16 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
       }
     }
     ld_kk += 64;
-    if (ld_kk == K) { ld_kk = 0; ld_tile += W; }
+    if (ld_kk == K) {
+      ld_kk = 0;
+      ld_tile = next_tile(ld_tile);
+    }
   };
   StreamSlot<RT> ring[DEPTH];
 #pragma unroll
   for (int s = 0; s < DEPTH - 1; ++s)
     if (s < nch) issue(ring[s]);
-  int64_t cur_tile = w;
+  int64_t cur_tile = first_tile;
   int cur_kk = 0;
   floatx4 d[RT];
 #pragma unroll
@@ -736,11 +797,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
     stream_step<PREC, RT>(sl.b, Ct, Ch, Cl, LDF, LDB, cur_kk + 32, c, g, d);
     cur_kk += 64;
     if (cur_kk == K) {
-      stream_finish_tile<PREC, XT, RT, EPI>(args, cur_tile, c, g, sl, wr, d, acc);
+      stream_finish_tile<PREC, XT, RT, EPI>(args, cur_tile, c, g, sl, wr, d, acc, FjT, LDJ);
 #pragma unroll
       for (int nb = 0; nb < RT; ++nb) d[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
       cur_kk = 0;
-      cur_tile += W;
+      cur_tile = next_tile(cur_tile);
     }
   };
   for (int64_t i = 0; i < nch; i += DEPTH) {
@@ -893,8 +954,7 @@ __global__ __launch_bounds__(512) void mttkrp_last_kernel(LastArgs args) {
           const bool ok = (p0 + 8 * g + v) < pe && r < R;
           w8[v] = ok ? gm * args.Fj[(jj + v) * R + r] : 0.0f;
         }
-        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(w8, wh[mb], wl[mb]);
-        else wh[mb] = __builtin_convertvector(w8, bf16x8);
+        split_bf16(w8, wh[mb], wl[mb]);
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -907,10 +967,8 @@ __global__ __launch_bounds__(512) void mttkrp_last_kernel(LastArgs args) {
 #pragma unroll
         for (int mb = 0; mb < RT; ++mb) {
           acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xh, acc[t][mb]);
-          if constexpr (PREC == TLA_PREC_BF16X3) {
-            acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xl, acc[t][mb]);
-            acc[t][mb] = mfma_bf16_16x16x32(wl[mb], xh, acc[t][mb]);
-          }
+          acc[t][mb] = mfma_bf16_16x16x32(wl[mb], xh, acc[t][mb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xl, acc[t][mb]);
         }
       }
       jj += STEP;
@@ -1014,8 +1072,7 @@ __device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4
     bf16x8 wh[RT], wl[RT];
 #pragma unroll
     for (int mb = 0; mb < RT; ++mb) {
-      if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(w8[mb], wh[mb], wl[mb]);
-      else wh[mb] = __builtin_convertvector(w8[mb], bf16x8);
+      split_bf16(w8[mb], wh[mb], wl[mb]);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -1028,10 +1085,8 @@ __device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4
 #pragma unroll
       for (int mb = 0; mb < RT; ++mb) {
         acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xh, acc[t][mb]);
-        if constexpr (PREC == TLA_PREC_BF16X3) {
-          acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xl, acc[t][mb]);
-          acc[t][mb] = mfma_bf16_16x16x32(wl[mb], xh, acc[t][mb]);
-        }
+        acc[t][mb] = mfma_bf16_16x16x32(wl[mb], xh, acc[t][mb]);
+        if constexpr (PREC == TLA_PREC_BF16X3) acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xl, acc[t][mb]);
       }
     }
   }
@@ -1320,6 +1375,11 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
           blocks = Wv / NW;
         }
         p->s_W = Wv;
+        // one wave per q (all its tiles in a row) when every wave gets at
+        // least two values of q and F_j fits LDS beside the F_k panel
+        const size_t fjt = size_t(RT * 16) * (J + 4) * 4;
+        if (p->sepi == SE_REDQ && s.ntiles / tpq >= 2 * Wv && lds + fjt <= 150 * 1024)
+          p->sepi = SE_REDQG;
         if (p->sepi == SE_ACCJ) {
           s.na = ndim - 2; s.nb = 0; s.In = 1; s.Pb = 1;
           for (int m = 0; m < ndim - 2; ++m) s.asz[m] = shape[m];
@@ -1335,9 +1395,11 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
           p->t_Pa = Pa; p->t_In = In; p->t_Pb = Pbr; p->t_S = (int)tpq;
         }
         p->grid = dim3((unsigned)blocks, 1, 1);
-        p->smem = lds;
-        p->partial_elems = (p->sepi == SE_ROWS) ? 0
-                           : (p->sepi == SE_REDQ) ? size_t(s.ntiles) * R : size_t(Wv) * 16 * R;
+        p->smem = lds + (p->sepi == SE_REDQG ? fjt : 0);
+        p->partial_elems = (p->sepi == SE_ROWS)    ? 0
+                           : (p->sepi == SE_REDQ)  ? size_t(s.ntiles) * R
+                           : (p->sepi == SE_REDQG) ? size_t(s.ntiles / tpq) * R
+                                                   : size_t(Wv) * 16 * R;
         // block scales Sq[q, r] = w[r] * prod of the factor rows of q's digits
         // (the output digit excluded), one row per 16 * tpq-row block
         GmArgs& sq = p->gm;
@@ -1346,7 +1408,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
         if (p->sepi == SE_ACCJ) {
           sq.n = ndim - 2;
           for (int m = 0; m < ndim - 2; ++m) sq.sz[m] = shape[m];
-        } else if (p->sepi == SE_REDQ) {
+        } else if (p->sepi == SE_REDQ || p->sepi == SE_REDQG) {
           int n = 0;
           for (int m = 0; m < kb; ++m) sq.sz[n++] = shape[m];
           sq.sz[n++] = In;
@@ -1358,6 +1420,7 @@ static int make_plan(int x_dtype, int ndim, const int64_t* shape, int R, int kb,
         p->ws_bytes = align256(size_t(p->gm_rows) * R * 4) + align256(p->partial_elems * 4);
         p->red_off = p->ws_bytes;
         if (p->sepi == SE_REDQ) p->ws_bytes += align256(tiles_reduce_tmp_elems(Pa, In, p->t_Pb * tpq, R) * 4);
+        if (p->sepi == SE_REDQG) p->ws_bytes += align256(tiles_reduce_tmp_elems(Pa, In, p->t_Pb, R) * 4);
         return TLA_OK;
       }
     }
@@ -1581,6 +1644,7 @@ template <int PREC, typename XT>
 static int launch_stream(const Plan& p, hipStream_t st) {
   if (p.sepi == SE_ROWS) return launch_stream_epi<PREC, XT, SE_ROWS>(p, st);
   if (p.sepi == SE_REDQ) return launch_stream_epi<PREC, XT, SE_REDQ>(p, st);
+  if (p.sepi == SE_REDQG) return launch_stream_epi<PREC, XT, SE_REDQG>(p, st);
   return launch_stream_epi<PREC, XT, SE_ACCJ>(p, st);
 }
 
@@ -1728,6 +1792,8 @@ static int run_plan(const Plan& p0, int x_dtype, const void* X, int ndim, const 
     if (rc != TLA_OK || rows_out) return rc;
     if (p.sepi == SE_REDQ) {
       tiles_reduce(part, p.t_Pa, p.t_In, p.t_Pb * p.t_S, R, (float*)((char*)ws + p.red_off), (float*)out, st);
+    } else if (p.sepi == SE_REDQG) {
+      tiles_reduce(part, p.t_Pa, p.t_In, p.t_Pb, R, (float*)((char*)ws + p.red_off), (float*)out, st);
     } else {
       const int64_t n = s.J * R;
       (void)n;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void ttm_rows_kernel(TtmRowsArgs a) {
       } else {
         const __bf16 h = (__bf16)v;
         Ch[n * LDB + kk] = h;
-        if constexpr (PREC == TLA_PREC_BF16X3) Cl[n * LDB + kk] = (__bf16)(v - (float)h);
+        Cl[n * LDB + kk] = (__bf16)(v - (float)h);
       }
     }
     __syncthreads();
@@ -146,12 +146,10 @@ __global__ __launch_bounds__(256) void ttm_rows_kernel(TtmRowsArgs a) {
         for (int nb = 0; nb < RT; ++nb) {
           const int off = (nb * 16 + c) * LDB + kk + 8 * g;
           const bf16x8 bh = *(const bf16x8*)(Ch + off);
+          const bf16x8 bl = *(const bf16x8*)(Cl + off);
           d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
-          if constexpr (PREC == TLA_PREC_BF16X3) {
-            const bf16x8 bl = *(const bf16x8*)(Cl + off);
-            d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
-            d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
-          }
+          d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) d[nb] = mfma_bf16_16x16x32(al, bh, d[nb]);
         }
       }
     }
@@ -236,8 +234,7 @@ __global__ __launch_bounds__(256) void ttm_cols_kernel(TtmColsArgs a) {
           const int64_t m = m0 + 8 * g + v;
           u8[v] = (m < me && j < a.J) ? a.U[j * a.ldj + m * a.ldm] : 0.0f;
         }
-        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(u8, uh[mb], ul[mb]);
-        else uh[mb] = __builtin_convertvector(u8, bf16x8);
+        split_bf16(u8, uh[mb], ul[mb]);
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -250,10 +247,8 @@ __global__ __launch_bounds__(256) void ttm_cols_kernel(TtmColsArgs a) {
 #pragma unroll
         for (int mb = 0; mb < RT; ++mb) {
           acc[t][mb] = mfma_bf16_16x16x32(uh[mb], xh, acc[t][mb]);
-          if constexpr (PREC == TLA_PREC_BF16X3) {
-            acc[t][mb] = mfma_bf16_16x16x32(uh[mb], xl, acc[t][mb]);
-            acc[t][mb] = mfma_bf16_16x16x32(ul[mb], xh, acc[t][mb]);
-          }
+          acc[t][mb] = mfma_bf16_16x16x32(ul[mb], xh, acc[t][mb]);
+          if constexpr (PREC == TLA_PREC_BF16X3) acc[t][mb] = mfma_bf16_16x16x32(uh[mb], xl, acc[t][mb]);
         }
       }
     }

@@ -124,6 +124,31 @@ def test_mttkrp_bf16_copy(cuda, shape, rank):
         assert err < 1e-2, (mode, err)
 
 
+@pytest.mark.parametrize("shape", [(16, 32, 64), (8, 16, 24, 64), (40, 24, 96), (64, 64, 64, 64)])
+@pytest.mark.parametrize("rank", [16, 33, 64])
+@pytest.mark.parametrize("copy", [False, True])
+def test_mttkrp_bf16_exact_for_rounded_tensor(cuda, shape, rank, copy):
+    """BF16 multiplies the bf16-rounded tensor (rounded on the fly, or a
+    bf16 copy) by the factors split in two bf16 terms: against the FP64
+    oracle MTTKRP of the rounded tensor it is as accurate as bf16x3 (the
+    split factors keep 16 of FP32's 24 bits), on signed data, for every
+    kernel family these shapes reach (row-tile, last-mode, order 4)."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(zlib.crc32(repr((shape, rank, copy)).encode()))
+    X = rng.standard_normal(shape).astype(np.float32)
+    f = [rng.standard_normal((s, rank)).astype(np.float32) for s in shape]
+    Xg = g(X, cuda, torch.float32)
+    Xr = Xg.to(torch.bfloat16)
+    X64 = Xr.double().cpu().numpy()
+    fg = [g(x, cuda, torch.float32) for x in f]
+    with tla.precision("bf16"):
+        for mode in range(len(shape)):
+            ref = T.unfolding_dot_khatri_rao(X64, (None, [x.astype(np.float64) for x in f]), mode)
+            out = tla.unfolding_dot_khatri_rao(Xr if copy else Xg, (None, fg), mode)
+            err = rel(out.cpu(), ref)
+            assert err < 2e-5, (mode, err)
+
+
 @pytest.mark.parametrize("fj", ["0", "1"])
 @pytest.mark.parametrize("rank", [5, 16, 32, 64])
 def test_last_mode_fj_staging(cuda, monkeypatch, fj, rank):
