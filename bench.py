@@ -691,10 +691,30 @@ def main():
                                   ("mttkrp_mode1_fp32", "fp32", "mttkrp", 1),
                                   ("mttkrp_mode2_fp32", "fp32", "mttkrp", 2)):
         kern[name] = kernel_time_ms(raw_launcher(X, facs, prec, kind, arg, torch), 20, torch)
+    # the same kernels reading a bf16 copy of the tensor (BF16: the rounded
+    # tensor times split factors); half the tensor bytes
+    Xb = X.to(torch.bfloat16)
+    for name, kind, arg in (("mttkrp_partial_modes01_bf16copy", "partial", (0, 2)),
+                            ("mttkrp_mode2_bf16copy", "mttkrp", 2)):
+        kern[name] = kernel_time_ms(raw_launcher(Xb, facs, "bf16", kind, arg, torch), 20, torch)
     partial_bytes = 512 ** 3 * 4 + 512 * RANK * 4 + 512 * 512 * RANK * 4
-    kernels = {k: {"ms": round(v, 4),
-                   "GB/s": round((partial_bytes if "partial" in k else MTTKRP_BYTES) / (v * 1e-3) / 1e9, 1)}
-               for k, v in kern.items()}
+    half = 512 ** 3 * 2  # tensor bytes saved by the bf16 copy
+
+    def kbytes(k):
+        b = partial_bytes if "partial" in k else MTTKRP_BYTES
+        return b - half if "bf16copy" in k else b
+    kernels = {k: {"ms": round(v, 4), "GB/s": round(kbytes(k) / (v * 1e-3) / 1e9, 1)} for k, v in kern.items()}
+    # the BF16 dimension-tree sweep on the bf16 copy
+    bfe = CpAlsEngine(X, ones, init, precision="bf16", dimtree=True, unit_weights=True, tensor_bf16=Xb)
+    bfe.sweeps(args.warmup, None, 0, False)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter()
+    bfe.sweeps(args.steps, None, 0, False)
+    torch.cuda.synchronize()
+    sweep_bf16 = {"ms": round((time.perf_counter() - t_b) / args.steps * 1e3, 4), "dimtree": bfe.dimtree,
+                  "note": "BF16 on a bf16 copy of the tensor (the rounded tensor times split factors)"}
+    bfe.close()
+    log(f"[bench] BF16 (bf16 copy) dimension-tree sweep: {sweep_bf16['ms']} ms")
     # the roofline kernel: the dimension-tree partial contraction is ONE kernel
     # launch (mttkrp_stream_kernel, row epilogue) and the largest per sweep
     dom = "mttkrp_partial_modes01_bf16x3"
@@ -720,6 +740,19 @@ def main():
                 res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
                 eng.close()
                 log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
+            # BF16 on the bf16 copy: the target is checked on the driver's
+            # error (of the rounded tensor); the final factors' error against
+            # the float32 tensor is reported beside it
+            from tensorly_amd.decomposition._cp import error_calc
+            eng = CpAlsEngine(X, ones, init, precision="bf16", dimtree=True, unit_weights=True, tensor_bf16=Xb)
+            t, n, e = time_to_fit(eng, args.fit_sweeps, target, torch)
+            nx = torch.sqrt(torch.sum(X.double() ** 2))
+            with tla.precision("fp32"):
+                true_err = float(error_calc(X, nx, ones, eng.factors)[0] / nx)
+            res["bf16copy_dimtree"] = {"seconds": t, "sweeps": n, "final_error": float(e[-1]),
+                                       "final_error_vs_float32_tensor": true_err}
+            eng.close()
+            log(f"[bench] time-to-fit bf16copy_dimtree: {res['bf16copy_dimtree']}")
             res["bf16x3_dimtree_fixed_alpha4"] = learned_time_to_fit(X, ones, init, None, args.fit_sweeps, target,
                                                                      torch, alpha_fixed=4.0)
             log(f"[bench] time-to-fit bf16x3_dimtree_fixed_alpha4: {res['bf16x3_dimtree_fixed_alpha4']}")
@@ -771,6 +804,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},
         "kernels": kernels,
+        "sweep_bf16_copy": sweep_bf16,
         "time_to_fit": ttf,
         "batched": batched,
         "cpu_baseline": cpu,

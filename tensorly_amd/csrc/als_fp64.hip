@@ -137,29 +137,43 @@ __device__ __forceinline__ doublex4 mfma_f64(double a, double b, doublex4 c) {
 
 static_assert(kSolveRows == 64, "the solve's MFMA tiling assigns 16 rows to each of 4 waves");
 
-template <typename T>
+// RPC: the padded rank (16, 32 or 64) as a compile-time constant, so the
+// MFMA k-loops unroll and their LDS operands are requested ahead.
+template <typename T, int RPC>
 __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   __shared__ double Lbuf[kMaxRank * kMaxRank];  // V, then the rounded new rows (kSolveRows x RP)
   __shared__ double Vi[kMaxRank * kMaxRank];    // V^-1, RP x RP
   __shared__ double Mr[kSolveRows * kMaxRank];  // this block's MTTKRP rows, kSolveRows x RP, zero padded
   __shared__ double pbuf[6 * kMaxRank];
   __shared__ int flag, piv, ticket, pflag[2];
-  const int R = a.R, RP = solve_pad(R), RT = RP / 16, tid = threadIdx.x;
+  constexpr int RP = RPC, RT = RPC / 16;
+  const int R = a.R, tid = threadIdx.x;
   constexpr int wdt = sizeof(T) == 8 ? TLA_F64 : TLA_F32;
   const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
   const int64_t r0 = (int64_t)blockIdx.x * kSolveRows;
   const int nr = (int)min((int64_t)kSolveRows, a.rows - r0);
-  for (int e = tid; e < kSolveRows * RP; e += blockDim.x) {
-    const int i = e / RP, c = e % RP;
+  // Every global load of the prologue in one batch: the MTTKRP rows, the
+  // precomputed V^-1 (copied whether or not its status says it is valid;
+  // an invalid one is overwritten below) and that status.
+  const int pre_status = a.vi_pre ? *a.vi_status : -1;  // the same word for every thread
+#pragma unroll
+  for (int e0 = 0; e0 < kSolveRows * RP; e0 += 256) {
+    const int e = e0 + tid, i = e / RP, c = e % RP;
     Mr[e] = (i < nr && c < R) ? ld<T>(a.mttkrp, (r0 + i) * R + c) : 0.0;
   }
-  int status = a.vi_pre ? *a.vi_status : -1;  // the same word for every thread
-  if (status >= 0) {
-    for (int e = tid; e < RP * RP; e += blockDim.x) {
-      const int r = e / RP, s = e % RP;
-      Vi[e] = (r < R && s < R) ? a.vi_pre[r * R + s] : 0.0;
+  if (a.vi_pre) {
+#pragma unroll
+    for (int e0 = 0; e0 < RP * RP; e0 += 256) {
+      const int e = e0 + tid;
+      if (e < RP * RP) {
+        const int r = e / RP, s = e % RP;
+        Vi[e] = (r < R && s < R) ? a.vi_pre[r * R + s] : 0.0;
+      }
     }
-  } else {
+  }
+  int status = pre_status;
+  if (status < 0) {
+    __syncthreads();  // the speculative copy is done before invert_v reuses Vi
     status = invert_v<256>(a, a.ndim, a.mode, -1, nullptr, a.weights, wdt, a.l2, R, Lbuf, Vi, pbuf, pflag, &piv, &flag);
   }
   if (blockIdx.x == 0 && tid == 0) {
@@ -176,6 +190,7 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
     doublex4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
     for (int k0 = 0; k0 < RP; k0 += 4) {
       const double av = Mr[(16 * w + lr) * RP + k0 + lk];
 #pragma unroll
@@ -204,6 +219,7 @@ __global__ __launch_bounds__(256) void cp_solve_kernel(SolveArgs a) {
   for (int tile = w; tile < RT * RT; tile += 4) {
     const int tr = tile / RT, tc = tile % RT;
     doublex4 g = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
     for (int k0 = 0; k0 < kSolveRows; k0 += 4)
       g = mfma_f64(An[(k0 + lk) * RP + 16 * tr + lr], An[(k0 + lk) * RP + 16 * tc + lr], g);
 #pragma unroll
@@ -670,8 +686,16 @@ int tla::cp_solve_launch(int dtype, const double* const* grams, int ndim, int mo
   a.vi_next = vi_next;
   a.vist_next = vist_next;
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == TLA_F32) cp_solve_kernel<float><<<nblk, 256, 0, st>>>(a);
-  else cp_solve_kernel<double><<<nblk, 256, 0, st>>>(a);
+  const int rp = solve_pad(rank);
+#define TLA_SOLVE_CASE(RPV)                                                 \
+  if (rp == RPV) {                                                          \
+    if (dtype == TLA_F32) cp_solve_kernel<float, RPV><<<nblk, 256, 0, st>>>(a);  \
+    else cp_solve_kernel<double, RPV><<<nblk, 256, 0, st>>>(a);               \
+  }
+  TLA_SOLVE_CASE(16)
+  TLA_SOLVE_CASE(32)
+  TLA_SOLVE_CASE(64)
+#undef TLA_SOLVE_CASE
   TLA_LAUNCH_CHECK();
   if (!counter) {
     gram_finalize_kernel<<<1, 256, 0, st>>>(a.gram_partial, nblk, rank, gram_out, a.ip_partial, iprod_out);
