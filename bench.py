@@ -435,7 +435,8 @@ def load_model(torch, dev, path=None):
     return ExtrapolationMLP(np.load(path)["params"], device=dev)
 
 
-def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch, alpha_fixed=None):
+def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch, alpha_fixed=None, precision="bf16x3",
+                        tensor_bf16=None):
     """bf16x3 dimension-tree sweeps with the learned step (or, with
     ``alpha_fixed``, a fixed step and no network): find the first sweep slot
     whose error reaches the target, then time a fresh run of exactly that
@@ -446,7 +447,7 @@ def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch, alpha_fi
         pol = LearnedExtrapolation(None, alpha_fixed=alpha_fixed, alpha_max=16.0, warmup=3)
     else:
         pol = LearnedExtrapolation(net, alpha_max=16.0, warmup=3)
-    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
+    eng = CpAlsEngine(X, ones, init, precision=precision, dimtree=True, unit_weights=True, tensor_bf16=tensor_bf16)
     errs, acc, alphas, _ = run_extrapolated_sweeps(eng, n_slots, pol)
     e = errs.cpu().numpy()
     a = acc.cpu().numpy()
@@ -458,7 +459,7 @@ def learned_time_to_fit(X, ones, init, net, n_slots, target_err, torch, alpha_fi
         out.update({"seconds": None, "sweeps": None})
         return out
     k = int(hit[0]) + 1
-    eng = CpAlsEngine(X, ones, init, precision="bf16x3", dimtree=True, unit_weights=True)
+    eng = CpAlsEngine(X, ones, init, precision=precision, dimtree=True, unit_weights=True, tensor_bf16=tensor_bf16)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -740,19 +741,34 @@ def main():
                 res[name] = {"seconds": t, "sweeps": n, "final_error": float(e[-1])}
                 eng.close()
                 log(f"[bench] time-to-fit {name}: {t} s in {n} sweeps (final err {e[-1]:.8f}, target {target:.8f})")
-            # BF16 on the bf16 copy: the target is checked on the driver's
-            # error (of the rounded tensor); the final factors' error against
-            # the float32 tensor is reported beside it
+            # BF16 on the bf16 copy: the driver's errors are the rounded
+            # tensor's, so the sweep that reaches the target is found from
+            # each sweep's factors' error against the float32 tensor (FP32
+            # MTTKRP, FP64 sums), then a fresh run of that many sweeps is timed
             from tensorly_amd.decomposition._cp import error_calc
-            eng = CpAlsEngine(X, ones, init, precision="bf16", dimtree=True, unit_weights=True, tensor_bf16=Xb)
-            t, n, e = time_to_fit(eng, args.fit_sweeps, target, torch)
             nx = torch.sqrt(torch.sum(X.double() ** 2))
-            with tla.precision("fp32"):
-                true_err = float(error_calc(X, nx, ones, eng.factors)[0] / nx)
-            res["bf16copy_dimtree"] = {"seconds": t, "sweeps": n, "final_error": float(e[-1]),
-                                       "final_error_vs_float32_tensor": true_err}
+            eng = CpAlsEngine(X, ones, init, precision="bf16", dimtree=True, unit_weights=True, tensor_bf16=Xb)
+            snaps = []
+            for _ in range(args.fit_sweeps):
+                eng.sweeps(1, None, 0, False)
+                snaps.append([f.clone() for f in eng.factors])
             eng.close()
-            log(f"[bench] time-to-fit bf16copy_dimtree: {res['bf16copy_dimtree']}")
+            with tla.precision("fp32"):
+                true_errs = [float(error_calc(X, nx, ones, f)[0] / nx) for f in snaps]
+            del snaps
+            hit = [i for i, e_ in enumerate(true_errs) if e_ <= target]
+            row = {"final_error_vs_float32_tensor": true_errs[-1], "seconds": None, "sweeps": None}
+            if hit:
+                k = hit[0] + 1
+                eng = CpAlsEngine(X, ones, init, precision="bf16", dimtree=True, unit_weights=True, tensor_bf16=Xb)
+                torch.cuda.synchronize()
+                t_k = time.perf_counter()
+                eng.sweeps(k, None, 0, False)
+                torch.cuda.synchronize()
+                row.update({"seconds": time.perf_counter() - t_k, "sweeps": k, "error_at_hit": true_errs[k - 1]})
+                eng.close()
+            res["bf16copy_dimtree"] = row
+            log(f"[bench] time-to-fit bf16copy_dimtree: {row}")
             res["bf16x3_dimtree_fixed_alpha4"] = learned_time_to_fit(X, ones, init, None, args.fit_sweeps, target,
                                                                      torch, alpha_fixed=4.0)
             log(f"[bench] time-to-fit bf16x3_dimtree_fixed_alpha4: {res['bf16x3_dimtree_fixed_alpha4']}")

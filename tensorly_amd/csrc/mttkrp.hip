@@ -551,14 +551,19 @@ __device__ __forceinline__ void scale_q(const StreamArgs& a, int64_t q, int c, f
   }
 }
 
+// The prefetch slots keep 8 tensor values as loaded (float, or bf16 for a
+// bf16 copy): a conversion at load time would make the wave wait for the
+// load right away and defeat the prefetch.
+template <typename XT> struct StreamVec { using type = floatx8; };
+template <> struct StreamVec<__bf16> { using type = bf16x8; };
+
 // 8 consecutive tensor values at p (16-byte aligned); NT: non-temporal
 // (streaming) loads, which measured faster for the kernels whose output is
 // small, and slower for the T01 partial that writes 33.5 MB while it reads.
 template <typename XT, bool NT>
-__device__ __forceinline__ floatx8 load_stream_x8(const XT* p) {
+__device__ __forceinline__ typename StreamVec<XT>::type load_stream_x8(const XT* p) {
   if constexpr (sizeof(XT) == 2) {
-    const bf16x8 v = NT ? __builtin_nontemporal_load((const bf16x8*)p) : *(const bf16x8*)p;
-    return __builtin_convertvector(v, floatx8);
+    return NT ? __builtin_nontemporal_load((const bf16x8*)p) : *(const bf16x8*)p;
   } else {
     const floatx4 lo = NT ? __builtin_nontemporal_load((const floatx4*)p) : *(const floatx4*)p;
     const floatx4 hi = NT ? __builtin_nontemporal_load((const floatx4*)(p + 4)) : *(const floatx4*)(p + 4);
@@ -570,9 +575,25 @@ __device__ __forceinline__ floatx8 load_stream_x8(const XT* p) {
 }
 
 template <typename XT, bool NT = false>
-__device__ __forceinline__ void load_pair(const XT* p, floatx8& v0, floatx8& v1) {
+__device__ __forceinline__ void load_pair(const XT* p, typename StreamVec<XT>::type& v0,
+                                          typename StreamVec<XT>::type& v1) {
   v0 = load_stream_x8<XT, NT>(p);
   v1 = load_stream_x8<XT, NT>(p + 32);
+}
+
+// A bf16 copy (BF16 only): the loaded values are the A operand as they are.
+template <int PREC, int RT>
+__device__ __forceinline__ void stream_step(const bf16x8& ah, const float* Ct, const __bf16* Ch, const __bf16* Cl,
+                                            int LDF, int LDB, int kk, int c, int g, floatx4 d[RT]) {
+  static_assert(PREC == TLA_PREC_BF16, "a bf16 tensor copy runs BF16");
+#pragma unroll
+  for (int nb = 0; nb < RT; ++nb) {
+    const int off = (nb * 16 + c) * LDB + kk + 8 * g;
+    const bf16x8 bh = *(const bf16x8*)(Ch + off);
+    const bf16x8 bl = *(const bf16x8*)(Cl + off);
+    d[nb] = mfma_bf16_16x16x32(ah, bh, d[nb]);
+    d[nb] = mfma_bf16_16x16x32(ah, bl, d[nb]);
+  }
 }
 
 template <int PREC, int RT>
@@ -607,16 +628,16 @@ constexpr int kStreamWaves = 8;  // 512-thread blocks share one LDS copy of F_k
 // the tile's last, the tile's epilogue operands (loaded in the same batch so
 // the in-order return of vector loads never makes the epilogue wait on the
 // prefetched chunks behind it).
-template <int RT>
+template <int RT, typename XT>
 struct StreamSlot {
-  floatx8 a, b;
+  typename StreamVec<XT>::type a, b;
   float fj[4][RT];
   float sq[RT];
 };
 
 template <int PREC, typename XT, int RT, int EPI>
 __device__ __forceinline__ void stream_finish_tile(const StreamArgs& args, int64_t tile, int c, int g,
-                                                   const StreamSlot<RT>& sl, const float wr[RT], floatx4 d[RT],
+                                                   const StreamSlot<RT, XT>& sl, const float wr[RT], floatx4 d[RT],
                                                    floatx4 acc[RT], const float* FjT, int LDJ) {
   const int R = args.R;
   const int64_t row0 = tile * 16;
@@ -746,7 +767,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   // multiplied.
   int64_t ld_tile = first_tile;
   int ld_kk = 0;
-  auto issue = [&](StreamSlot<RT>& sl) {
+  auto issue = [&](StreamSlot<RT, XT>& sl) {
     load_pair<XT, EPI != SE_ROWS>(X + (ld_tile * 16 + c) * args.K + ld_kk + 8 * g, sl.a, sl.b);
     if (ld_kk + 64 == K) {
       if constexpr (EPI == SE_REDQG) {
@@ -783,7 +804,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
       ld_tile = next_tile(ld_tile);
     }
   };
-  StreamSlot<RT> ring[DEPTH];
+  StreamSlot<RT, XT> ring[DEPTH];
 #pragma unroll
   for (int s = 0; s < DEPTH - 1; ++s)
     if (s < nch) issue(ring[s]);
@@ -792,7 +813,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(Stream
   floatx4 d[RT];
 #pragma unroll
   for (int nb = 0; nb < RT; ++nb) d[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-  auto consume = [&](const StreamSlot<RT>& sl) {
+  auto consume = [&](const StreamSlot<RT, XT>& sl) {
     stream_step<PREC, RT>(sl.a, Ct, Ch, Cl, LDF, LDB, cur_kk, c, g, d);
     stream_step<PREC, RT>(sl.b, Ct, Ch, Cl, LDF, LDB, cur_kk + 32, c, g, d);
     cur_kk += 64;
@@ -1022,23 +1043,31 @@ struct LastStreamArgs {
 
 // One visit's operands: 8 rows of this lane's column quad plus the Khatri-Rao
 // row factors of those rows (requested together, see StreamSlot).
-template <int RT>
+template <typename XT> struct LastVec { using type = floatx4; };
+template <> struct LastVec<__bf16> { using type = bf16x4; };
+template <int RT, typename XT>
 struct LastSlot {
-  floatx4 x[8];
+  typename LastVec<XT>::type x[8];  // as loaded (see StreamVec)
   float gm[RT];
   float fj[8][RT];
 };
 
+// 4 consecutive tensor values at p, as loaded, non-temporal
+template <typename XT>
+__device__ __forceinline__ typename LastVec<XT>::type load_raw4_nt(const XT* p) {
+  return __builtin_nontemporal_load((const typename LastVec<XT>::type*)p);
+}
+
 template <typename XT, int RT>
 __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* X, int64_t col, bool colok,
-                                          int64_t rowb, int64_t pe, int c, LastSlot<RT>& sl) {
+                                          int64_t rowb, int64_t pe, int c, LastSlot<RT, XT>& sl) {
   const int R = args.R;
   const int64_t K = args.K, J = args.J;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int64_t row = rowb + u;
-    sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-    if (row < pe && colok) sl.x[u] = load_x4_nt<XT>(X + row * K + col);
+    sl.x[u] = typename LastVec<XT>::type{};
+    if (row < pe && colok) sl.x[u] = load_raw4_nt<XT>(X + row * K + col);
   }
   // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0
   const bool okrow = rowb < pe;
@@ -1053,8 +1082,8 @@ __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* 
   }
 }
 
-template <int PREC, int RT>
-__device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4][RT]) {
+template <int PREC, int RT, typename XT>
+__device__ __forceinline__ void last_visit(const LastSlot<RT, XT>& sl, floatx4 acc[4][RT]) {
   floatx8 w8[RT];
 #pragma unroll
   for (int mb = 0; mb < RT; ++mb)
@@ -1076,12 +1105,18 @@ __device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      floatx8 x8;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) x8[u] = sl.x[u][t];
       bf16x8 xh, xl;
-      if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(x8, xh, xl);
-      else xh = __builtin_convertvector(x8, bf16x8);
+      if constexpr (sizeof(XT) == 2) {
+        // a bf16 copy: column t of the 8 rows, as stored
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xh[u] = sl.x[u][t];
+      } else {
+        floatx8 x8;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x8[u] = sl.x[u][t];
+        if constexpr (PREC == TLA_PREC_BF16X3) split_bf16(x8, xh, xl);
+        else xh = __builtin_convertvector(x8, bf16x8);
+      }
 #pragma unroll
       for (int mb = 0; mb < RT; ++mb) {
         acc[t][mb] = mfma_bf16_16x16x32(wh[mb], xh, acc[t][mb]);
@@ -1095,23 +1130,23 @@ __device__ __forceinline__ void last_visit(const LastSlot<RT>& sl, floatx4 acc[4
 // LDS variant: the slot holds only the tensor rows and the Gm row; the
 // F_j rows are read from the transposed LDS copy at visit time, which keeps
 // two slots and 4 rank tiles of accumulators within the register file.
-template <int RT>
+template <int RT, typename XT>
 struct LastSlotL {
-  floatx4 x[8];
+  typename LastVec<XT>::type x[8];
   float gm[RT];
   int jj;
 };
 
 template <typename XT, int RT>
 __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* X, int64_t col, bool colok,
-                                          int64_t rowb, int64_t pe, int c, LastSlotL<RT>& sl) {
+                                          int64_t rowb, int64_t pe, int c, LastSlotL<RT, XT>& sl) {
   const int R = args.R;
   const int64_t K = args.K, J = args.J;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int64_t row = rowb + u;
-    sl.x[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-    if (row < pe && colok) sl.x[u] = load_x4_nt<XT>(X + row * K + col);
+    sl.x[u] = typename LastVec<XT>::type{};
+    if (row < pe && colok) sl.x[u] = load_raw4_nt<XT>(X + row * K + col);
   }
   // rows rowb + u share aa because J % 8 == 0 and rowb % 8 == 0; a row
   // block past the end reads F_j row 0 (its x and gm are zero)
@@ -1125,10 +1160,10 @@ __device__ __forceinline__ void last_load(const LastStreamArgs& args, const XT* 
   }
 }
 
-template <int PREC, int RT>
-__device__ __forceinline__ void last_visit(const LastSlotL<RT>& sl, const float* fjs, int ldT, int c,
+template <int PREC, int RT, typename XT>
+__device__ __forceinline__ void last_visit(const LastSlotL<RT, XT>& sl, const float* fjs, int ldT, int c,
                                            floatx4 acc[4][RT]) {
-  LastSlot<RT> t;
+  LastSlot<RT, XT> t;
 #pragma unroll
   for (int u = 0; u < 8; ++u) t.x[u] = sl.x[u];
 #pragma unroll
@@ -1143,7 +1178,7 @@ __device__ __forceinline__ void last_visit(const LastSlotL<RT>& sl, const float*
       t.fj[u + 4][mb] = f1[u];
     }
   }
-  last_visit<PREC, RT>(t, acc);
+  last_visit<PREC, RT, XT>(t, acc);
 }
 
 template <int PREC, typename XT, int RT, bool FJL>
@@ -1181,7 +1216,7 @@ __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs 
   // visit v covers rows v * V + 32 * rs .. +31; lane rows p0 + 8 g + u.
   auto rowb_of = [&](int64_t v) { return v * V + 32 * rs + 8 * g; };
   auto pe_of = [&](int64_t v) { return min(args.P, v * V + 32 * rs + 32); };
-  using Slot = typename std::conditional<FJL, LastSlotL<RT>, LastSlot<RT>>::type;
+  using Slot = typename std::conditional<FJL, LastSlotL<RT, XT>, LastSlot<RT, XT>>::type;
   const int ldT = args.ldT;
   Slot sa, sb;
   int64_t va = b;
@@ -1189,13 +1224,13 @@ __global__ __launch_bounds__(512) void mttkrp_last_stream_kernel(LastStreamArgs 
   for (; va < args.nvisits; va += 2 * args.G) {
     const int64_t vb = va + args.G;
     if (vb < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vb), pe_of(vb), c, sb);
-    if constexpr (FJL) last_visit<PREC, RT>(sa, fjs, ldT, c, acc);
-    else last_visit<PREC, RT>(sa, acc);
+    if constexpr (FJL) last_visit<PREC, RT, XT>(sa, fjs, ldT, c, acc);
+    else last_visit<PREC, RT, XT>(sa, acc);
     if (vb >= args.nvisits) break;
     const int64_t vc = vb + args.G;
     if (vc < args.nvisits) last_load<XT, RT>(args, X, col, colok, rowb_of(vc), pe_of(vc), c, sa);
-    if constexpr (FJL) last_visit<PREC, RT>(sb, fjs, ldT, c, acc);
-    else last_visit<PREC, RT>(sb, acc);
+    if constexpr (FJL) last_visit<PREC, RT, XT>(sb, fjs, ldT, c, acc);
+    else last_visit<PREC, RT, XT>(sb, acc);
   }
   // acc[t][mb][e] = out[col + t][mb * 16 + 4 g + e]
   const int64_t slot = b * args.rsplit + rs;

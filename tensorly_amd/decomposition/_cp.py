@@ -186,8 +186,10 @@ class CpAlsEngine:
                                       _lib.ctypes.byref(handle)), "cp_als_create")
         self.handle = handle
         if xcode == _lib.BF16:
-            # ‖X‖² of the float32 original (the bf16 copy only feeds MTTKRP)
-            nx2 = torch.sum(tensor.to(torch.float64) ** 2).reshape(1)
+            # ‖X‖² of the bf16 copy: the run solves the rounded tensor's
+            # problem, and its errors are that problem's (the float32
+            # original's differ by about the rounding's ‖X - X_bf16‖)
+            nx2 = torch.sum(self.x.to(torch.float64) ** 2).reshape(1)
             self.refresh(with_norm=False)
             check(lib().tla_cp_als_set_normx2(self.handle, ptr(nx2), stream(self.dev)), "cp_als_set_normx2")
         else:
