@@ -340,6 +340,12 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
     # the device-side sweep loop normalises float32 factors only
     host_loop = (bool(tol) or callback is not None or linesearch or verbose or bool(orthogonalise)
                  or imputer is not None or (normalize and tensor.dtype != torch.float32))
+    # only the stopping rule needs the host: check it once per chunk of
+    # sweeps and roll back to the stopping sweep's factors
+    chunked = (bool(tol) and callback is None and not linesearch and not verbose and not orthogonalise
+               and imputer is None and not normalize)
+    if chunked:
+        return _run_chunked(eng, n_iter_max, tol, return_errors, cvg_criterion, errs_dev)
     norm_tensor = torch.sqrt(eng.normx2())
 
     if callback is not None:
@@ -425,6 +431,43 @@ def _run_plain(eng, tensor, n_iter_max, tol, return_errors, normalize, cvg_crite
         if normalize:
             w, f = cp_normalize((eng.weights, eng.factors))
             eng.set_factors(w, f)
+    return _result(eng, rec_errors, return_errors)
+
+
+# sweeps per host check of the stopping rule
+_TOL_CHUNK = 8
+
+
+def _run_chunked(eng, n_iter_max, tol, return_errors, cvg_criterion, errs_dev):
+    """The plain ALS loop with ``tol`` (``_cp.py:484-514``) with one host
+    synchronisation per ``_TOL_CHUNK`` sweeps instead of one per sweep: the
+    sweeps of a chunk run back to back on the device, each followed by a copy
+    of the factors; the stopping rule is then applied sweep by sweep on the
+    recorded errors, and at the stopping sweep the factors are restored from
+    its copy, so the result is the one the per-sweep check gives."""
+    if cvg_criterion not in ("abs_rec_error", "rec_error"):
+        raise TypeError("Unknown convergence criterion")
+    rec_errors = []
+    it = 0
+    while it < n_iter_max:
+        k = min(_TOL_CHUNK, n_iter_max - it)
+        snaps = []
+        for j in range(k):
+            eng.sweeps(1, errs_dev, it + j, False)
+            snaps.append([f.clone() for f in eng.factors])
+        errs = [float(v) for v in errs_dev[it:it + k].cpu()]
+        eng.raise_if_singular()
+        for j in range(k):
+            rec_errors.append(errs[j])
+            i = it + j
+            if i >= 1:
+                dec = rec_errors[-2] - rec_errors[-1]
+                stop = abs(dec) < tol if cvg_criterion == "abs_rec_error" else dec < tol
+                if stop:
+                    if j < k - 1:
+                        eng.set_factors(eng.weights, snaps[j])
+                    return _result(eng, rec_errors, return_errors)
+        it += k
     return _result(eng, rec_errors, return_errors)
 
 

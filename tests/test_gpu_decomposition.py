@@ -49,6 +49,32 @@ def test_parafac_float32_vs_oracle(cuda, prec, dimtree):
     assert np.max(np.abs(np.array(errs) - np.array(ref_errs))) < 1e-5, (prec, dimtree)
 
 
+# Seeds and tolerances where the FP64 oracle stops at sweep 9, 17 and 30, with
+# the last two error decreases at least 1.5 % of tol on either side of it.
+@pytest.mark.parametrize("tol,crit,seed", [(1e-3, "rec_error", 2), (1e-3, "abs_rec_error", 6),
+                                           (3e-4, "rec_error", 1), (1e-7, "abs_rec_error", 6)])
+def test_parafac_tol_chunked_matches_per_sweep_check(cuda, tol, crit, seed):
+    """With only ``tol`` set, parafac checks the stopping rule once per chunk
+    of sweeps and rolls back to the stopping sweep; a callback forces the
+    per-sweep host check.  Both stop at the same sweep with the same bits,
+    and, for tolerances the oracle reaches within 60 sweeps, at the sweep
+    where the FP64 oracle stops."""
+    import tensorly_amd as tla
+    rng = np.random.default_rng(8)
+    f_true = [rng.random((s, 5)) for s in (40, 36, 52)]
+    x = synthetic.add_noise(synthetic.cp_model(np.ones(5), f_true, dtype=np.float32), 0.01, rng)
+    xg = g(x, cuda, torch.float32)
+    kw = dict(n_iter_max=60, init="random", random_state=seed, tol=tol, cvg_criterion=crit, return_errors=True)
+    cp_a, errs_a = tla.parafac(xg, 5, **kw)
+    cp_b, errs_b = tla.parafac(xg, 5, callback=lambda cp, e: False, **kw)
+    assert errs_a == errs_b
+    for fa, fb in zip(cp_a.factors, cp_b.factors):
+        assert torch.equal(fa, fb)
+    if tol >= 1e-4:
+        (_, _), ref_errs = T.parafac(x.astype(np.float64), 5, **kw)
+        assert len(errs_a) == len(ref_errs) < 60, (len(errs_a), len(ref_errs))
+
+
 @pytest.mark.parametrize("shape", [(64, 48, 128), (16, 32, 16, 64)])
 @pytest.mark.parametrize("prec,dimtree", [("fp32", False), ("bf16x3", True), ("fp32", True)])
 @pytest.mark.parametrize("rank", [8, 40])
