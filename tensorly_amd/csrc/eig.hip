@@ -114,6 +114,71 @@ __device__ void block_gemm_f64_panel(const double* A, const double* B, int ldb, 
   __syncthreads();
 }
 
+// Z = G V for the zero-padded symmetric np x np G and the np x B block V
+// (both in global memory, ld np and B), read as G^T V so that each A-operand
+// load takes 4 rows x 128 contiguous bytes of G.  V goes through LDS (vs,
+// KC x B doubles) in KC-row chunks; each wave keeps two 16-row strips of Z
+// with all B / 16 column tiles in registers, so one pass reads every element
+// of G once and every V operand comes from LDS.  Same k-ascending MFMA
+// sequence per output tile as block_gemm_f64.  Ends with a barrier.
+template <int B, int KC>
+__device__ void gv_product(const double* __restrict__ G, int np, const double* __restrict__ V, double* Z,
+                           double* vs) {
+  constexpr int TN = B / 16, SPW = 2;
+  static_assert(B % 16 == 0 && KC % 32 == 0, "tile shape");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const int nw = blockDim.x >> 6, nstrip = np / 16;
+  for (int g0 = 0; g0 < nstrip; g0 += nw * SPW) {
+    const int s0 = g0 + wave * SPW;
+    const bool live0 = s0 < nstrip, live1 = s0 + 1 < nstrip;  // wave-uniform
+    doublex4 acc[SPW][TN];
+#pragma unroll
+    for (int s = 0; s < SPW; ++s)
+#pragma unroll
+      for (int t = 0; t < TN; ++t) acc[s][t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int kc = 0; kc < np; kc += KC) {
+      const int kn = min(KC, np - kc);  // a multiple of 16
+      __syncthreads();                  // the previous chunk is consumed
+      for (int e = threadIdx.x; e < kn * B; e += blockDim.x) vs[e] = V[(int64_t)kc * B + e];
+      __syncthreads();
+      if (live0) {
+        const double* g0p = G + (int64_t)kc * np + s0 * 16 + lr;
+        int k0 = 0;
+        for (; k0 + 16 <= kn; k0 += 16) {
+          double a0[4], a1[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kk = k0 + 4 * q + lk;
+            a0[q] = g0p[(int64_t)kk * np];
+            a1[q] = live1 ? g0p[(int64_t)kk * np + 16] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kk = k0 + 4 * q + lk;
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+              const double bv = vs[kk * B + t * 16 + lr];
+              acc[0][t] = mfma_f64(a0[q], bv, acc[0][t]);
+              acc[1][t] = mfma_f64(a1[q], bv, acc[1][t]);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      const int strip = s0 + s;
+      if (strip < nstrip) {
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) Z[(int64_t)(strip * 16 + lk + 4 * e) * B + t * 16 + lr] = acc[s][t][e];
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Jacobi on an m x m symmetric matrix A (LDS, leading dimension LDA: odd, so
 // that the column walks of a rotation hit distinct banks), accumulating the
@@ -134,11 +199,66 @@ __device__ __forceinline__ double jac_rsqrt(double x) {
   return y * fma(-hx * y, y, 1.5);
 }
 
+// Slot a's pair (p < q) in round `round` of the round-robin ordering of me
+// (even) slots: slot 0 stays, the others rotate.
+__device__ __forceinline__ void jac_pair(int a, int round, int me, int& p, int& q) {
+  const int L = me - 1;
+  int x = (a == 0 ? me - 2 : a - 1) + round;
+  if (x >= L) x -= L;
+  int y = me - 2 - a + round;
+  if (y >= L) y -= L;
+  p = a == 0 ? 0 : 1 + x;
+  q = a == 0 ? 1 + x : 1 + y;
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+// W <- W J for one round's rotations (cr: c, s per slot), items e = first,
+// first + step, ...: each item is a (row, slot) piece, four in flight.
+template <int LDW>
+__device__ __forceinline__ void jac_apply_w(double* W, const double* cr, int m, int me, int round, int first,
+                                            int step) {
+  const int npairs = me / 2, items = npairs * m;
+  for (int e0 = first; e0 < items; e0 += 4 * step) {
+    double x[4], y[4], c[4], s[4];
+    int p[4], q[4], i[4];
+    bool on[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = e0 + u * step;
+      const int pb = e % npairs;
+      i[u] = e / npairs;
+      jac_pair(pb, round, me, p[u], q[u]);
+      on[u] = e < items && q[u] < m;  // the dummy slot has no column
+      if (on[u]) {
+        c[u] = cr[2 * pb];
+        s[u] = cr[2 * pb + 1];
+        x[u] = W[i[u] * LDW + p[u]];
+        y[u] = W[i[u] * LDW + q[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (on[u]) {
+        W[i[u] * LDW + p[u]] = c[u] * x[u] - s[u] * y[u];
+        W[i[u] * LDW + q[u]] = s[u] * x[u] + c[u] * y[u];
+      }
+  }
+}
+
+// One sweep is me - 1 rounds of me / 2 disjoint rotations.  Per round: wave 0
+// computes the round's rotations while the other waves apply the previous
+// round's to W; after a barrier every thread applies A <- J^T A J (whole 2 x 2
+// blocks, two in flight).  cs holds 2 x me doubles (two rounds' c, s).
 template <int LDA, int LDW>
-__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, int* pr, double* scratch,
+__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, double* scratch,
                           double rel_off2 = 1e-30) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int me = m + (m & 1);  // even number of slots (one dummy)
+  const int npairs = me / 2;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     // off-diagonal and total Frobenius norms
@@ -162,74 +282,80 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
     __syncthreads();
     if (!(scratch[32] > rel_off2 * scratch[33]) || scratch[33] == 0.0) break;
     for (int round = 0; round < me - 1; ++round) {
-      // round-robin pairing: slot 0 fixed, others rotate
-      if (tid < me / 2) {
-        const int a = tid;
-        int p = (a == 0) ? 0 : 1 + (a - 1 + round) % (me - 1);
-        int q = 1 + (me - 1 - a - 1 + round) % (me - 1);
-        if (a == 0) q = 1 + (me - 2 + round) % (me - 1);
-        if (p > q) { const int t = p; p = q; q = t; }
-        double c = 1.0, s = 0.0;
-        if (q < m) {
-          const double apq = A[p * LDA + q];
-          if (fabs(apq) > 1e-300) {
-            // tan of the rotation angle t = sgn(tau) / (|tau| + sqrt(1 + tau^2)),
-            // c = 1 / sqrt(1 + t^2); reciprocals and reciprocal square roots
-            // from the hardware estimates refined by Newton steps (this
-            // is on the critical path of every round)
-            const double app = A[p * LDA + p], aqq = A[q * LDA + q];
-            const double tau = (aqq - app) * jac_recip(2.0 * apq);
-            const double at = fabs(tau);
-            double t;
-            if (at > 1e100) {
-              t = 0.5 / at;
-            } else {
-              const double u = 1.0 + tau * tau;
-              t = jac_recip(at + u * jac_rsqrt(u));
+      double* cr = cs + (round & 1) * me;
+      if (tid < 64) {
+        if (tid < npairs) {
+          int p, q;
+          jac_pair(tid, round, me, p, q);
+          double c = 1.0, s = 0.0;
+          if (q < m) {
+            const double apq = A[p * LDA + q], app = A[p * LDA + p], aqq = A[q * LDA + q];
+            if (fabs(apq) > 1e-300) {
+              // tan of the rotation angle t = sgn(tau) / (|tau| + sqrt(1 + tau^2)),
+              // c = 1 / sqrt(1 + t^2); reciprocals and reciprocal square roots
+              // from the hardware estimates refined by Newton steps (this
+              // is on the critical path of every round)
+              const double tau = (aqq - app) * jac_recip(2.0 * apq);
+              const double at = fabs(tau);
+              double t;
+              if (at > 1e100) {
+                t = 0.5 / at;
+              } else {
+                const double u = 1.0 + tau * tau;
+                t = jac_recip(at + u * jac_rsqrt(u));
+              }
+              t = tau >= 0.0 ? t : -t;
+              c = jac_rsqrt(1.0 + t * t);
+              s = t * c;
             }
-            t = tau >= 0.0 ? t : -t;
-            c = jac_rsqrt(1.0 + t * t);
-            s = t * c;
           }
+          cr[2 * tid] = c;
+          cr[2 * tid + 1] = s;
         }
-        pr[2 * a] = p;
-        pr[2 * a + 1] = q;
-        cs[2 * a] = c;
-        cs[2 * a + 1] = s;
+      } else if (round > 0) {
+        jac_apply_w<LDW>(W, cs + ((round - 1) & 1) * me, m, me, round - 1, tid - 64, nt - 64);
       }
       __syncthreads();
-      // A <- J^T A J and W <- W J in one pass: each thread owns whole 2 x 2
-      // blocks (rows of pair pa, columns of pair pb) and whole (row, pair)
-      // pieces of W, so no entry is touched twice
-      const int npairs = me / 2;
-      for (int e = tid; e < npairs * npairs; e += nt) {
-        const int pa = e / npairs, pb = e % npairs;
-        const int p0 = pr[2 * pa], q0 = pr[2 * pa + 1], p1 = pr[2 * pb], q1 = pr[2 * pb + 1];
-        const double ca = cs[2 * pa], sa = cs[2 * pa + 1], cb = cs[2 * pb], sb = cs[2 * pb + 1];
-        const bool r1 = q0 < m, c1 = q1 < m;  // the dummy slot has no row / column
-        const double a00 = A[p0 * LDA + p1];
-        const double a01 = c1 ? A[p0 * LDA + q1] : 0.0;
-        const double a10 = r1 ? A[q0 * LDA + p1] : 0.0;
-        const double a11 = (r1 && c1) ? A[q0 * LDA + q1] : 0.0;
-        const double u00 = ca * a00 - sa * a10, u01 = ca * a01 - sa * a11;
-        const double u10 = sa * a00 + ca * a10, u11 = sa * a01 + ca * a11;
-        A[p0 * LDA + p1] = cb * u00 - sb * u01;
-        if (c1) A[p0 * LDA + q1] = sb * u00 + cb * u01;
-        if (r1) A[q0 * LDA + p1] = cb * u10 - sb * u11;
-        if (r1 && c1) A[q0 * LDA + q1] = sb * u10 + cb * u11;
-      }
-      for (int e = tid; e < npairs * m; e += nt) {
-        const int pb = e % npairs, i = e / npairs;
-        const int p = pr[2 * pb], q = pr[2 * pb + 1];
-        if (q < m) {
-          const double c = cs[2 * pb], s = cs[2 * pb + 1];
-          const double x = W[i * LDW + p], y = W[i * LDW + q];
-          W[i * LDW + p] = c * x - s * y;
-          W[i * LDW + q] = s * x + c * y;
+      // A <- J^T A J: each thread owns whole 2 x 2 blocks (rows of pair pa,
+      // columns of pair pb), so no entry is touched twice
+      for (int e0 = tid; e0 < npairs * npairs; e0 += 2 * nt) {
+        double v[2][4], ca[2], sa[2], cb[2], sb[2];
+        int p0[2], q0[2], p1[2], q1[2];
+        bool on[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int e = e0 + u * nt;
+          on[u] = e < npairs * npairs;
+          const int pa = e / npairs, pb = e % npairs;
+          jac_pair(pa, round, me, p0[u], q0[u]);
+          jac_pair(pb, round, me, p1[u], q1[u]);
+          if (on[u]) {
+            const bool r1 = q0[u] < m, c1 = q1[u] < m;  // the dummy slot has no row / column
+            ca[u] = cr[2 * pa]; sa[u] = cr[2 * pa + 1];
+            cb[u] = cr[2 * pb]; sb[u] = cr[2 * pb + 1];
+            v[u][0] = A[p0[u] * LDA + p1[u]];
+            v[u][1] = c1 ? A[p0[u] * LDA + q1[u]] : 0.0;
+            v[u][2] = r1 ? A[q0[u] * LDA + p1[u]] : 0.0;
+            v[u][3] = (r1 && c1) ? A[q0[u] * LDA + q1[u]] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (!on[u]) continue;
+          const bool r1 = q0[u] < m, c1 = q1[u] < m;
+          const double u00 = ca[u] * v[u][0] - sa[u] * v[u][2], u01 = ca[u] * v[u][1] - sa[u] * v[u][3];
+          const double u10 = sa[u] * v[u][0] + ca[u] * v[u][2], u11 = sa[u] * v[u][1] + ca[u] * v[u][3];
+          A[p0[u] * LDA + p1[u]] = cb[u] * u00 - sb[u] * u01;
+          if (c1) A[p0[u] * LDA + q1[u]] = sb[u] * u00 + cb[u] * u01;
+          if (r1) A[q0[u] * LDA + p1[u]] = cb[u] * u10 - sb[u] * u11;
+          if (r1 && c1) A[q0[u] * LDA + q1[u]] = sb[u] * u10 + cb[u] * u11;
         }
       }
       __syncthreads();
     }
+    // the last round's rotations into W
+    jac_apply_w<LDW>(W, cs + ((me - 2) & 1) * me, m, me, me - 2, tid, nt);
+    __syncthreads();
   }
   return sweep;
 }
@@ -283,6 +409,36 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j) {
   return (double)h * (1.0 / 4294967296.0) - 0.5;
 }
 
+// Set-up of the subspace problems of one launch (blockIdx.y selects), spread
+// over many workgroups: the zero-padded symmetric copy Gp = (G + G^T) / 2 in
+// 32 x 32 tiles (the transposed tile of G read row by row through LDS), then
+// the starting block Y (np x B): warm-start columns, then pseudo-random ones.
+template <int B>
+__global__ __launch_bounds__(256) void eig_prep_kernel(EigBatch batch, int gtiles) {
+  const EigArgs a = batch.a[blockIdx.y];
+  const int n = a.n, np = a.np, tid = threadIdx.x;
+  if ((int)blockIdx.x < gtiles) {
+    __shared__ double Tt[32][33];
+    const int nt = (np + 31) / 32;
+    if ((int)blockIdx.x >= nt * nt) return;
+    const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+    for (int e = tid; e < 1024; e += 256) {  // Tt[r][c] = G[tj * 32 + r][ti * 32 + c]
+      const int r = e >> 5, c = e & 31, gi = tj * 32 + r, gj = ti * 32 + c;
+      Tt[r][c] = (gi < n && gj < n) ? a.G[(int64_t)gi * n + gj] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 256) {
+      const int r = e >> 5, c = e & 31, i = ti * 32 + r, j = tj * 32 + c;
+      if (i < np && j < np) a.Gp[(int64_t)i * np + j] = (i < n && j < n) ? 0.5 * (a.G[(int64_t)i * n + j] + Tt[c][r]) : 0.0;
+    }
+    return;
+  }
+  const int64_t e = (int64_t)(blockIdx.x - gtiles) * 256 + tid;
+  if (e >= (int64_t)np * B) return;
+  const int i = (int)(e / B), j = (int)(e % B);
+  a.Y[e] = i >= n ? 0.0 : ((a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j));
+}
+
 // Jacobi on the whole matrix: CAP 64 keeps A and W in LDS; CAP 128 keeps A
 // in LDS and W in the workspace (a.W, ld 128).
 template <int CAP>
@@ -291,8 +447,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
   constexpr int LDA = CAP + 1;
   __shared__ double A[CAP * LDA];
   __shared__ double Wl[CAP == kJac ? CAP * LDA : 1];
-  __shared__ double cs[CAP + 2];
-  __shared__ int pr[CAP + 2];
+  __shared__ double cs[2 * (CAP + 2)];
   __shared__ double scratch[64];
   __shared__ int order[CAP];
   constexpr int LDW = CAP == kJac ? LDA : CAP;
@@ -304,7 +459,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
     W[i * LDW + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, pr, scratch);
+  const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, scratch);
   sort_desc<LDA>(A, n, order);
   for (int e = tid; e < n * a.k; e += blockDim.x) {
     const int i = e / a.k, r = e % a.k;
@@ -390,8 +545,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   __shared__ double H[B * LDH];  // Ritz matrix; also S of the orthonormalisation; then sorted rotations
   __shared__ double Wl[SMALL ? B * LDH : 1];
   __shared__ double Tl[SMALL ? B * LDH : 1];
-  __shared__ double cs[B + 2];
-  __shared__ int pr[B + 2];
+  __shared__ double cs[2 * (B + 2)];
   __shared__ double scratch[64];
   __shared__ double resid[B];
   __shared__ int order[B];
@@ -409,16 +563,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   double* Z = a.Z;
   double* Y = a.Y;
   double* U = a.U;
-  // zero-padded copy of G; V0: warm start columns, then pseudo-random ones
-  for (int e = tid; e < np * np; e += nt) {
-    const int i = e / np, j = e % np;
-    a.Gp[e] = (i < n && j < n) ? 0.5 * (a.G[i * n + j] + a.G[j * n + i]) : 0.0;
-  }
-  for (int e = tid; e < np * B; e += nt) {
-    const int i = e / B, j = e % B;
-    Y[i * CAP + j] = i >= n ? 0.0 : ((a.Q0 && j < a.k0) ? a.Q0[(int64_t)i * a.q0_ld + j] : hash_unit(i, j));
-  }
-  __syncthreads();
+  // Gp (zero-padded symmetric G) and the starting block Y come from
+  // eig_prep_kernel, launched just before on the same stream
   if (a.Q0 && a.k0 > 0) {
     // orthonormal warm-start columns and B - k0 pseudo-random ones: the
     // block is well conditioned (cond(S) ~ 1e2), one unshifted Cholesky-QR
@@ -438,7 +584,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // Z = G V, read as G^T V (Gp is exactly symmetric): each load then takes
     // 4 rows x 128 contiguous bytes of G instead of 16 rows x 32 bytes, which
     // measured 2.3x faster
-    block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
+    if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
+    else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
     block_gemm_f64<true>(V, CAP, Z, CAP, H, LDH, B, B, np);      // H = V^T Z
     for (int e = tid; e < B * B; e += nt) {
       const int i = e / B, j = e % B;
@@ -449,7 +596,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // the Ritz rotations only need the accuracy of the current subspace:
     // off-diagonal mass down to (residual / t_1)^2 / 100 of the total,
     // tightening to tol^2 / 100 as the residuals fall
-    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch, jthr);
+    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, scratch, jthr);
     sort_desc<LDH>(H, B, order);
     if (tid < B) theta[tid] = H[order[tid] * LDH + order[tid]];
     __syncthreads();
@@ -686,6 +833,22 @@ static int eig_launch(const EigArgs* args, const int* kinds, int count, hipStrea
     for (int i = 0; i < count; ++i)
       if (kinds[i] == kind) b.a[m++] = args[i];
     if (m == 0) continue;
+    if (kind >= 2) {
+      int gt = 0, yt = 0;
+      const int bk = kind == 2 ? 48 : (kind == 3 ? 64 : kEigCapLarge);
+      for (int i = 0; i < m; ++i) {
+        const int nt = (b.a[i].np + 31) / 32;
+        gt = std::max(gt, nt * nt);
+        yt = std::max(yt, (int)ceil_div((int64_t)b.a[i].np * bk, 256));
+      }
+      dim3 grid(gt + yt, m);
+      switch (kind) {
+        case 2: eig_prep_kernel<48><<<grid, 256, 0, st>>>(b, gt); break;
+        case 3: eig_prep_kernel<64><<<grid, 256, 0, st>>>(b, gt); break;
+        default: eig_prep_kernel<kEigCapLarge><<<grid, 256, 0, st>>>(b, gt); break;
+      }
+      TLA_LAUNCH_CHECK();
+    }
     switch (kind) {
       case 0: sym_eig_small_kernel<kJac><<<m, kEigThreads, 0, st>>>(b); break;
       case 1: sym_eig_small_kernel<kJacLarge><<<m, kEigThreads, 0, st>>>(b); break;
