@@ -216,55 +216,128 @@ __device__ __forceinline__ void jac_pair(int a, int round, int me, int& p, int& 
   }
 }
 
-// W <- W J for one round's rotations (cr: c, s per slot), items e = first,
-// first + step, ...: each item is a (row, slot) piece, four in flight.
+// Items e = first, first + step, ... < count of an (outer, inner) grid with
+// `inner` columns, walked without divisions: (o, i) advances by (dq, dr).
+struct JacIter {
+  int o, i, dq, dr, inner;
+  __device__ JacIter(int first, int step, int inner_) : inner(inner_) {
+    o = first / inner_;
+    i = first - o * inner_;
+    dq = step / inner_;
+    dr = step - dq * inner_;
+  }
+  __device__ __forceinline__ void next() {
+    o += dq;
+    i += dr;
+    if (i >= inner) {
+      i -= inner;
+      ++o;
+    }
+  }
+};
+
+// W <- W J for one round (cr: (c, s) per slot, pr: p | q << 16 per slot; a
+// column index >= ma is the dummy slot): items (row < rows, slot) from
+// `first` by `step` (it: the walk from `first`), four in flight.
 template <int LDW>
-__device__ __forceinline__ void jac_apply_w(double* W, const double* cr, int m, int me, int round, int first,
-                                            int step) {
-  const int npairs = me / 2, items = npairs * m;
+__device__ __forceinline__ void jac_apply_w(double* W, const double2* cr, const int* pr, int rows, int ma, int npairs,
+                                            int first, int step, JacIter it) {
+  const int items = npairs * rows;
   for (int e0 = first; e0 < items; e0 += 4 * step) {
-    double x[4], y[4], c[4], s[4];
-    int p[4], q[4], i[4];
+    double x[4], y[4];
+    double2 cs[4];
+    int p[4], q[4], row[4];
     bool on[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int e = e0 + u * step;
-      const int pb = e % npairs;
-      i[u] = e / npairs;
-      jac_pair(pb, round, me, p[u], q[u]);
-      on[u] = e < items && q[u] < m;  // the dummy slot has no column
+      on[u] = e0 + u * step < items;
+      row[u] = it.o;
+      const int pq = on[u] ? pr[it.i] : 0;
+      p[u] = pq & 0xffff;
+      q[u] = pq >> 16;
+      on[u] = on[u] && q[u] < ma;  // the dummy slot has no column
       if (on[u]) {
-        c[u] = cr[2 * pb];
-        s[u] = cr[2 * pb + 1];
-        x[u] = W[i[u] * LDW + p[u]];
-        y[u] = W[i[u] * LDW + q[u]];
+        cs[u] = cr[it.i];
+        x[u] = W[row[u] * LDW + p[u]];
+        y[u] = W[row[u] * LDW + q[u]];
       }
+      it.next();
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (on[u]) {
-        W[i[u] * LDW + p[u]] = c[u] * x[u] - s[u] * y[u];
-        W[i[u] * LDW + q[u]] = s[u] * x[u] + c[u] * y[u];
+        W[row[u] * LDW + p[u]] = cs[u].x * x[u] - cs[u].y * y[u];
+        W[row[u] * LDW + q[u]] = cs[u].y * x[u] + cs[u].x * y[u];
       }
   }
 }
 
 // One sweep is me - 1 rounds of me / 2 disjoint rotations.  Per round: wave 0
-// computes the round's rotations while the other waves apply the previous
-// round's to W; after a barrier every thread applies A <- J^T A J (whole 2 x 2
-// blocks, two in flight).  cs holds 2 x me doubles (two rounds' c, s).
+// computes the round's rotations (c, s and the pair p | q << 16 per slot, in
+// cs / pr, two rounds' worth each) while the other waves apply the previous
+// round's to W; after a barrier every thread applies A <- J^T A J to whole
+// 2 x 2 blocks (two in flight).  Every thread's items are fixed for the call
+// and walked without divisions.
+//
+// lead > 0: only the eigenpairs of the `lead` largest eigenvalues are wanted.
+// If the Gershgorin discs of rows < lead lie above those of the other rows
+// (so the lead largest eigenvalues belong to the leading rows) and the
+// coupling between the two sets is below the threshold, the sweeps rotate
+// the leading lead x lead block only; the coupling and the trailing block are
+// left as they are (W must be the identity on entry; its trailing columns
+// stay so).  Returns the number of sweeps.
 template <int LDA, int LDW>
-__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* cs, double* scratch,
-                          double rel_off2 = 1e-30) {
+__device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double2* cs, int* pr, double* scratch,
+                          double rel_off2 = 1e-30, int lead = 0) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int me = m + (m & 1);  // even number of slots (one dummy)
-  const int npairs = me / 2;
+  int ma = m;  // the rows and columns the sweeps rotate
+  if (lead > 0 && lead < m) {
+    // per row: Gershgorin radius, squared coupling to the other set, squares
+    double lo = INFINITY, hi = -INFINITY, cpl = 0.0, tot = 0.0;
+    if (tid < m) {
+      double rad = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double v = A[tid * LDA + j];
+        tot += v * v;
+        if (j != tid) rad += fabs(v);
+        if ((j < lead) != (tid < lead)) cpl += v * v;
+      }
+      const double d = A[tid * LDA + tid];
+      if (tid < lead) lo = d - rad;
+      else hi = d + rad;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off, 64));
+      hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    cpl = wave_sum(cpl);
+    tot = wave_sum(tot);
+    if ((tid & 63) == 0) {
+      const int w = tid >> 6;
+      scratch[w] = lo; scratch[8 + w] = hi; scratch[16 + w] = cpl; scratch[24 + w] = tot;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double l = INFINITY, h = -INFINITY, c = 0.0, t = 0.0;
+      for (int w = 0; w < nt / 64 && w < 8; ++w) {
+        l = fmin(l, scratch[w]); h = fmax(h, scratch[8 + w]); c += scratch[16 + w]; t += scratch[24 + w];
+      }
+      scratch[34] = (l > h && c <= rel_off2 * t) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (scratch[34] != 0.0) ma = lead;
+  }
+  const int me = ma + (ma & 1);  // even number of slots (one dummy)
+  const int npairs = me / 2, nblk = npairs * npairs;
+  const JacIter blk0(tid, nt, npairs);                         // A blocks (pa, pb)
+  const JacIter wside(tid < 64 ? 0 : tid - 64, nt - 64, npairs);  // W (row, slot), waves 1..
+  const JacIter wall(tid, nt, npairs);                         // W (row, slot), every thread
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
-    // off-diagonal and total Frobenius norms
+    // off-diagonal and total Frobenius norms of the rotated block
     double off = 0.0, tot = 0.0;
-    for (int e = tid; e < m * m; e += nt) {
-      const int i = e / m, j = e % m;
+    for (int e = tid; e < ma * ma; e += nt) {
+      const int i = e / ma, j = e % ma;
       const double v = A[i * LDA + j];
       tot += v * v;
       if (i != j) off += v * v;
@@ -282,13 +355,14 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
     __syncthreads();
     if (!(scratch[32] > rel_off2 * scratch[33]) || scratch[33] == 0.0) break;
     for (int round = 0; round < me - 1; ++round) {
-      double* cr = cs + (round & 1) * me;
+      double2* cr = cs + (round & 1) * npairs;
+      int* prr = pr + (round & 1) * npairs;
       if (tid < 64) {
         if (tid < npairs) {
           int p, q;
           jac_pair(tid, round, me, p, q);
           double c = 1.0, s = 0.0;
-          if (q < m) {
+          if (q < ma) {
             const double apq = A[p * LDA + q], app = A[p * LDA + p], aqq = A[q * LDA + q];
             if (fabs(apq) > 1e-300) {
               // tan of the rotation angle t = sgn(tau) / (|tau| + sqrt(1 + tau^2)),
@@ -309,52 +383,56 @@ __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double* c
               s = t * c;
             }
           }
-          cr[2 * tid] = c;
-          cr[2 * tid + 1] = s;
+          cr[tid] = double2{c, s};
+          prr[tid] = p | (q << 16);
         }
       } else if (round > 0) {
-        jac_apply_w<LDW>(W, cs + ((round - 1) & 1) * me, m, me, round - 1, tid - 64, nt - 64);
+        jac_apply_w<LDW>(W, cs + ((round - 1) & 1) * npairs, pr + ((round - 1) & 1) * npairs, ma, ma, npairs,
+                         tid - 64, nt - 64, wside);
       }
       __syncthreads();
       // A <- J^T A J: each thread owns whole 2 x 2 blocks (rows of pair pa,
       // columns of pair pb), so no entry is touched twice
-      for (int e0 = tid; e0 < npairs * npairs; e0 += 2 * nt) {
-        double v[2][4], ca[2], sa[2], cb[2], sb[2];
+      JacIter it = blk0;
+      for (int e0 = tid; e0 < nblk; e0 += 2 * nt) {
+        double v[2][4];
+        double2 ca[2], cb[2];
         int p0[2], q0[2], p1[2], q1[2];
         bool on[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          const int e = e0 + u * nt;
-          on[u] = e < npairs * npairs;
-          const int pa = e / npairs, pb = e % npairs;
-          jac_pair(pa, round, me, p0[u], q0[u]);
-          jac_pair(pb, round, me, p1[u], q1[u]);
+          on[u] = e0 + u * nt < nblk;
           if (on[u]) {
-            const bool r1 = q0[u] < m, c1 = q1[u] < m;  // the dummy slot has no row / column
-            ca[u] = cr[2 * pa]; sa[u] = cr[2 * pa + 1];
-            cb[u] = cr[2 * pb]; sb[u] = cr[2 * pb + 1];
+            const int pqa = prr[it.o], pqb = prr[it.i];
+            ca[u] = cr[it.o];
+            cb[u] = cr[it.i];
+            p0[u] = pqa & 0xffff; q0[u] = pqa >> 16;
+            p1[u] = pqb & 0xffff; q1[u] = pqb >> 16;
+            const bool r1 = q0[u] < ma, c1 = q1[u] < ma;  // the dummy slot has no row / column
             v[u][0] = A[p0[u] * LDA + p1[u]];
             v[u][1] = c1 ? A[p0[u] * LDA + q1[u]] : 0.0;
             v[u][2] = r1 ? A[q0[u] * LDA + p1[u]] : 0.0;
             v[u][3] = (r1 && c1) ? A[q0[u] * LDA + q1[u]] : 0.0;
           }
+          it.next();
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           if (!on[u]) continue;
-          const bool r1 = q0[u] < m, c1 = q1[u] < m;
-          const double u00 = ca[u] * v[u][0] - sa[u] * v[u][2], u01 = ca[u] * v[u][1] - sa[u] * v[u][3];
-          const double u10 = sa[u] * v[u][0] + ca[u] * v[u][2], u11 = sa[u] * v[u][1] + ca[u] * v[u][3];
-          A[p0[u] * LDA + p1[u]] = cb[u] * u00 - sb[u] * u01;
-          if (c1) A[p0[u] * LDA + q1[u]] = sb[u] * u00 + cb[u] * u01;
-          if (r1) A[q0[u] * LDA + p1[u]] = cb[u] * u10 - sb[u] * u11;
-          if (r1 && c1) A[q0[u] * LDA + q1[u]] = sb[u] * u10 + cb[u] * u11;
+          const bool r1 = q0[u] < ma, c1 = q1[u] < ma;
+          const double sa = ca[u].y, cc = ca[u].x, sb = cb[u].y, cd = cb[u].x;
+          const double u00 = cc * v[u][0] - sa * v[u][2], u01 = cc * v[u][1] - sa * v[u][3];
+          const double u10 = sa * v[u][0] + cc * v[u][2], u11 = sa * v[u][1] + cc * v[u][3];
+          A[p0[u] * LDA + p1[u]] = cd * u00 - sb * u01;
+          if (c1) A[p0[u] * LDA + q1[u]] = sb * u00 + cd * u01;
+          if (r1) A[q0[u] * LDA + p1[u]] = cd * u10 - sb * u11;
+          if (r1 && c1) A[q0[u] * LDA + q1[u]] = sb * u10 + cd * u11;
         }
       }
       __syncthreads();
     }
     // the last round's rotations into W
-    jac_apply_w<LDW>(W, cs + ((me - 2) & 1) * me, m, me, me - 2, tid, nt);
+    jac_apply_w<LDW>(W, cs + ((me - 2) & 1) * npairs, pr + ((me - 2) & 1) * npairs, ma, ma, npairs, tid, nt, wall);
     __syncthreads();
   }
   return sweep;
@@ -447,7 +525,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
   constexpr int LDA = CAP + 1;
   __shared__ double A[CAP * LDA];
   __shared__ double Wl[CAP == kJac ? CAP * LDA : 1];
-  __shared__ double cs[2 * (CAP + 2)];
+  __shared__ double2 cs[CAP + 2];
+  __shared__ int pr[CAP + 2];
   __shared__ double scratch[64];
   __shared__ int order[CAP];
   constexpr int LDW = CAP == kJac ? LDA : CAP;
@@ -459,7 +538,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
     W[i * LDW + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, scratch);
+  const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, pr, scratch);
   sort_desc<LDA>(A, n, order);
   for (int e = tid; e < n * a.k; e += blockDim.x) {
     const int i = e / a.k, r = e % a.k;
@@ -545,7 +624,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   __shared__ double H[B * LDH];  // Ritz matrix; also S of the orthonormalisation; then sorted rotations
   __shared__ double Wl[SMALL ? B * LDH : 1];
   __shared__ double Tl[SMALL ? B * LDH : 1];
-  __shared__ double cs[2 * (B + 2)];
+  __shared__ double2 cs[B + 2];
+  __shared__ int pr[B + 2];
   __shared__ double scratch[64];
   __shared__ double resid[B];
   __shared__ int order[B];
@@ -596,7 +676,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // the Ritz rotations only need the accuracy of the current subspace:
     // off-diagonal mass down to (residual / t_1)^2 / 100 of the total,
     // tightening to tol^2 / 100 as the residuals fall
-    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, scratch, jthr);
+    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch, jthr, k);
     sort_desc<LDH>(H, B, order);
     if (tid < B) theta[tid] = H[order[tid] * LDH + order[tid]];
     __syncthreads();
