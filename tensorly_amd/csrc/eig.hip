@@ -660,6 +660,18 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
   double jthr = 1e-10;
   int it = 0;
+  if (a.Q0 && a.k0 > 0 && a.max_iter >= 2) {
+    // warm start: its first Ritz test fails whenever the matrix has moved by
+    // more than the tolerance, so the first step is a plain subspace step,
+    // V <- orth(G V) (the span a Rayleigh-Ritz step would leave), and the
+    // next Rayleigh-Ritz step rotates to the final threshold
+    if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
+    else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
+    block_orth<B>(Z, U, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+    block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    jthr = jfloor;
+    it = 1;
+  }
   for (; it < a.max_iter; ++it) {
     // Z = G V, read as G^T V (Gp is exactly symmetric): each load then takes
     // 4 rows x 128 contiguous bytes of G instead of 16 rows x 32 bytes, which
