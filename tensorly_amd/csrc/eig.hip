@@ -78,6 +78,48 @@ __device__ void block_gemm_f64(const double* A, int lda, const double* B, int ld
   __syncthreads();
 }
 
+// C (N x N) = A^T B for a product known to be symmetric (A^T A, or V^T G V
+// with G symmetric): only the tiles on and above the diagonal are formed
+// (16 k-steps' operands in flight) and each off-diagonal one is written to
+// both places (a diagonal tile holds whatever rounding its own sums give).
+// N a multiple of 16, K a multiple of 4.  Ends with a barrier.
+__device__ void block_gemm_sym_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int N,
+                                   int K) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const int tn = N / 16, ntile = tn * (tn + 1) / 2;
+  for (int t = wave; t < ntile; t += blockDim.x >> 6) {
+    int ti = 0, rem = t;  // t -> (ti <= tj), row by row of the upper triangle
+    while (rem >= tn - ti) {
+      rem -= tn - ti;
+      ++ti;
+    }
+    const int tj = ti + rem, mi = ti * 16, nj = tj * 16;
+    doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+    int k0 = 0;
+    for (; k0 + 64 <= K; k0 += 64) {
+      double a[16], b[16];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int kk = k0 + 4 * s + lk;
+        a[s] = A[kk * lda + mi + lr];
+        b[s] = B[kk * ldb + nj + lr];
+      }
+#pragma unroll
+      for (int s = 0; s < 16; ++s) acc = mfma_f64(a[s], b[s], acc);
+    }
+    for (; k0 < K; k0 += 4) {
+      const int kk = k0 + lk;
+      acc = mfma_f64(A[kk * lda + mi + lr], B[kk * ldb + nj + lr], acc);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      C[(mi + lk + 4 * e) * ldc + nj + lr] = acc[e];
+      if (ti != tj) C[(nj + lr) * ldc + mi + lk + 4 * e] = acc[e];
+    }
+  }
+  __syncthreads();
+}
+
 // C (M x N) = A B for a row-major A in global memory with lda == K == KB:
 // each wave stages one 16-row panel of A (16 * KB contiguous doubles, read
 // with coalesced loads) in LDS and uses it for all N / 16 column tiles.
@@ -560,7 +602,7 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
   // TPR threads per row of [S | I] (rows past B idle), CPT columns each
   constexpr int TPR = B > 64 ? 4 : 8, CPT = 2 * B / TPR;
   static_assert(TPR * B <= kEigThreads && CPT * TPR == 2 * B && TPR % 2 == 0, "thread layout");
-  block_gemm_f64<true>(M, CAP, M, CAP, Sbuf, LDS_S, B, B, np);
+  block_gemm_sym_f64(M, CAP, M, CAP, Sbuf, LDS_S, B, np);
   const int tid = threadIdx.x;
   if (tid == 0) {
     double mx = 0.0;
@@ -678,7 +720,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // measured 2.3x faster
     if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
     else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
-    block_gemm_f64<true>(V, CAP, Z, CAP, H, LDH, B, B, np);      // H = V^T Z
+    block_gemm_sym_f64(V, CAP, Z, CAP, H, LDH, B, np);           // H = V^T Z = V^T G V
     for (int e = tid; e < B * B; e += nt) {
       const int i = e / B, j = e % B;
       if (i < j) H[i * LDH + j] = H[j * LDH + i] = 0.5 * (H[i * LDH + j] + H[j * LDH + i]);
@@ -704,15 +746,26 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
       block_gemm_f64<false>(V, CAP, H, LDH, Y, CAP, np, B, B);
       block_gemm_f64<false>(Z, CAP, H, LDH, U, CAP, np, B, B);
     }
-    // residual norms of the leading k pairs (one wave per column)
-    for (int r = tid >> 6; r < k; r += nt >> 6) {
-      double r2 = 0.0;
-      for (int i = tid & 63; i < n; i += 64) {
-        const double d = U[i * CAP + r] - theta[r] * Y[i * CAP + r];
-        r2 += d * d;
+    // residual norms of the leading k pairs: thread (g, r) sums column r
+    // over rows g, g + ng, ... (row-wise, coalesced), partials through H
+    {
+      const int ng = nt / k;
+      if (tid < ng * k) {
+        const int g = tid / k, r = tid - g * k;
+        const double th = theta[r];
+        double r2 = 0.0;
+        for (int i = g; i < n; i += ng) {
+          const double d = U[i * CAP + r] - th * Y[i * CAP + r];
+          r2 += d * d;
+        }
+        H[tid] = r2;
       }
-      r2 = wave_sum(r2);
-      if ((tid & 63) == 0) resid[r] = sqrt(r2);
+      __syncthreads();
+      if (tid < k) {
+        double r2 = 0.0;
+        for (int g = 0; g < ng; ++g) r2 += H[g * k + tid];
+        resid[tid] = sqrt(r2);
+      }
     }
     __syncthreads();
     if (tid == 0) {
