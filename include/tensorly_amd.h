@@ -375,6 +375,25 @@ int tla_sym_eig_batch(int count, const double* const* G, const int* n, const int
                       const double* const* Q0, const int* k0, double* const* evals,
                       double* const* evecs, double tol, int max_iter, void* const* workspace,
                       const size_t* workspace_bytes, void* stream);
+/* tla_sym_eig with the rest of the reference's truncated-SVD path fused into
+ * the same launch: the k leading left singular vectors U (n x k, row-major,
+ * dtype TLA_F32 / TLA_F64) of a matrix whose row Gram is G, with the sign
+ * rule of svd_flip(u_based_decision=True) (tensorly/tenalg/svd.py:13-42:
+ * each column times the sign of its entry of largest magnitude, the first
+ * such row on ties), s = sqrt(max(eigenvalue, 0)) (k, dtype; nullable) and
+ * U64 (the same vectors in FP64; nullable).  For n > 128,
+ * tla_gram_svd_block(n, k) = b is the subspace block: W_out (n x b FP64,
+ * nullable) receives the final block, and W0 = that block with k0 = b (it
+ * may alias W_out) warm-starts a later call on a nearby matrix (the next
+ * HOOI step of the same mode, tensorly/decomposition/_tucker.py:194-202)
+ * without re-orthonormalising it; 0 < k0 < b columns are a partial warm
+ * start as for tla_sym_eig.  tla_gram_svd_block returns 0 where no block
+ * exists (n <= 128: Jacobi on the whole matrix, W0 and W_out unused).
+ * Workspace: tla_sym_eig_workspace_size(n). */
+int tla_gram_svd_block(int n, int k);
+int tla_gram_left_singular(const double* G, int n, int k, const double* W0, int k0, int dtype, void* U,
+                           void* s, double* U64, double* W_out, double tol, int max_iter, int* iters_out,
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Native CP-ALS sweep driver: the mode loop of

@@ -78,6 +78,9 @@ _SIGNATURES = {
     "tla_sym_eig_workspace_size": (c_sz, [c_i]),
     "tla_sym_eig": (c_i, [c_p, c_i, c_i, c_p, c_i, c_p, c_p, c_d, c_i, c_p, c_p, c_sz, c_p]),
     "tla_sym_eig_batch": (c_i, [c_i, PP, c_p, c_p, PP, c_p, PP, PP, c_d, c_i, PP, c_p, c_p]),
+    "tla_gram_svd_block": (c_i, [c_i, c_i]),
+    "tla_gram_left_singular": (c_i, [c_p, c_i, c_i, c_p, c_i, c_i, c_p, c_p, c_p, c_p, c_d, c_i, c_p, c_p, c_sz,
+                                     c_p]),
     "tla_cp_als_workspace_size": (c_sz, [c_i, c_i, P_i64, c_i, c_i, c_i]),
     "tla_cp_als_dimtree_supported": (c_i, [c_i, c_i, P_i64, c_i, c_i]),
     "tla_cp_als_create": (c_i, [c_i, c_p, c_i, P_i64, c_i, PP, c_p, c_d, c_i, c_i, ctypes.POINTER(c_i), c_p,

@@ -515,7 +515,72 @@ struct EigArgs {
   double* U;
   double* W;  // rotations when they do not fit LDS
   double* T;  // orthonormalising factor when it does not fit LDS
+  // singular-vector outputs (tla_gram_left_singular; zero / NULL otherwise)
+  int flip;        // apply the u-based sign rule to the k leading vectors
+  int out_dtype;   // TLA_F32 / TLA_F64 for u_out and s_out
+  void* u_out;     // n x k sign-corrected vectors in out_dtype
+  void* s_out;     // k square roots of max(eigenvalue, 0) in out_dtype
+  double* wblk;    // n x b: the whole final block (subspace kernel)
+  int wblock;      // Q0 is a whole block (k0 == b) used without orthonormalising
 };
+
+// Output of one problem: the k leading eigenpairs are column colmap[r] (r
+// without a map) of Ysrc (ld ldy) and theta[r].  evals / evecs as
+// tla_sym_eig writes them (when non-NULL); with a.flip the reference's
+// u-based sign rule first (each column multiplied by the sign of its entry
+// of largest magnitude, the first such row on ties; sgn: k doubles of LDS);
+// u_out / s_out in a.out_dtype; with a.wblk the first nb columns of Ysrc.
+__device__ void eig_emit(const EigArgs& a, const double* Ysrc, int ldy, const int* colmap, const double* theta,
+                         double* sgn, int nb) {
+  const int tid = threadIdx.x, nt = blockDim.x, n = a.n, k = a.k;
+  if (a.flip) {
+    for (int r = tid >> 6; r < k; r += nt >> 6) {
+      const int c = colmap ? colmap[r] : r;
+      double best = -1.0;
+      int bi = 0;
+      for (int i = tid & 63; i < n; i += 64) {
+        const double v = fabs(Ysrc[i * ldy + c]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if ((tid & 63) == 0) {
+        const double y = Ysrc[bi * ldy + c];
+        sgn[r] = y > 0.0 ? 1.0 : (y < 0.0 ? -1.0 : 0.0);
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * k; e += nt) {
+    const int i = e / k, r = e - (e / k) * k;
+    double v = Ysrc[i * ldy + (colmap ? colmap[r] : r)];
+    if (a.flip) v *= sgn[r];
+    if (a.evecs) a.evecs[e] = v;
+    if (a.u_out) {
+      if (a.out_dtype == TLA_F32) ((float*)a.u_out)[e] = (float)v;
+      else ((double*)a.u_out)[e] = v;
+    }
+  }
+  for (int r = tid; r < k; r += nt) {
+    if (a.evals) a.evals[r] = theta[r];
+    if (a.s_out) {
+      const double sv = sqrt(fmax(theta[r], 0.0));
+      if (a.out_dtype == TLA_F32) ((float*)a.s_out)[r] = (float)sv;
+      else ((double*)a.s_out)[r] = sv;
+    }
+  }
+  if (a.wblk)
+    for (int e = tid; e < n * nb; e += nt) a.wblk[e] = Ysrc[(e / nb) * ldy + e % nb];
+}
 
 // Problems solved by one launch, one workgroup each (blockIdx.x selects).
 constexpr int kEigMaxBatch = 8;
@@ -571,6 +636,8 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
   __shared__ int pr[CAP + 2];
   __shared__ double scratch[64];
   __shared__ int order[CAP];
+  __shared__ double theta[CAP];
+  __shared__ double sgn[CAP];
   constexpr int LDW = CAP == kJac ? LDA : CAP;
   double* W = CAP == kJac ? Wl : a.W;
   const int n = a.n, tid = threadIdx.x;
@@ -582,11 +649,9 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
   __syncthreads();
   const int sweeps = lds_jacobi<LDA, LDW>(A, W, n, 60, cs, pr, scratch);
   sort_desc<LDA>(A, n, order);
-  for (int e = tid; e < n * a.k; e += blockDim.x) {
-    const int i = e / a.k, r = e % a.k;
-    a.evecs[i * a.k + r] = W[i * LDW + order[r]];
-  }
-  for (int r = tid; r < a.k; r += blockDim.x) a.evals[r] = A[order[r] * LDA + order[r]];
+  for (int r = tid; r < a.k; r += blockDim.x) theta[r] = A[order[r] * LDA + order[r]];
+  __syncthreads();
+  eig_emit(a, W, LDW, order, theta, sgn, 0);
   if (tid == 0 && a.iters) *a.iters = sweeps;
 }
 
@@ -687,7 +752,13 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   double* U = a.U;
   // Gp (zero-padded symmetric G) and the starting block Y come from
   // eig_prep_kernel, launched just before on the same stream
-  if (a.Q0 && a.k0 > 0) {
+  if (a.Q0 && a.k0 > 0 && a.wblock) {
+    // a whole block from an earlier solve: the plain subspace step below
+    // orthonormalises G V, so V itself is used as it is
+    double* t = V;
+    V = Y;
+    Y = t;
+  } else if (a.Q0 && a.k0 > 0) {
     // orthonormal warm-start columns and B - k0 pseudo-random ones: the
     // block is well conditioned (cond(S) ~ 1e2), one unshifted Cholesky-QR
     // pass leaves V orthonormal to ~1e-13
@@ -786,11 +857,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
     block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
   }
-  for (int e = tid; e < n * k; e += nt) {
-    const int i = e / k, r = e % k;
-    a.evecs[i * k + r] = Y[i * CAP + r];
-  }
-  for (int r = tid; r < k; r += nt) a.evals[r] = theta[r];
+  eig_emit(a, Y, CAP, nullptr, theta, resid, B);
   if (tid == 0 && a.iters) *a.iters = it + 1;
 }
 
@@ -924,7 +991,7 @@ static int eig_args(const double* G, int n, int k, const double* Q0, int k0, dou
                     int max_iter, int* iters_out, void* ws, size_t ws_bytes, EigArgs* out, int* kind) {
   TLA_REQUIRE(n >= 1, "sym_eig: n = %d", n);
   TLA_REQUIRE(k >= 1 && k <= n, "sym_eig: k = %d not in [1, n = %d]", k, n);
-  TLA_REQUIRE(G && evals && evecs, "sym_eig: NULL pointer");
+  TLA_REQUIRE(G, "sym_eig: NULL pointer");
   TLA_REQUIRE(!Q0 || k0 >= 1, "sym_eig: warm start with k0 = %d columns", k0);
   TLA_REQUIRE(ws_bytes >= tla_sym_eig_workspace_size(n), "sym_eig: workspace too small");
   TLA_REQUIRE(n <= kJacLarge || ws, "sym_eig: NULL workspace");
@@ -1008,10 +1075,35 @@ static int eig_launch(const EigArgs* args, const int* kinds, int count, hipStrea
 
 extern "C" int tla_sym_eig(const double* G, int n, int k, const double* Q0, int k0, double* evals, double* evecs,
                            double tol, int max_iter, int* iters_out, void* ws, size_t ws_bytes, void* stream) {
+  TLA_REQUIRE(evals && evecs, "sym_eig: NULL pointer");
   EigArgs a;
   int kind = 0;
   const int rc = eig_args(G, n, k, Q0, k0, evals, evecs, tol, max_iter, iters_out, ws, ws_bytes, &a, &kind);
   if (rc != TLA_OK) return rc;
+  return eig_launch(&a, &kind, 1, (hipStream_t)stream);
+}
+
+extern "C" int tla_gram_svd_block(int n, int k) {
+  if (n <= kJacLarge || k < 1 || k > n || k > kEigCapLarge - 8) return 0;
+  return k <= 40 ? 48 : (k <= 56 ? 64 : kEigCapLarge);
+}
+
+extern "C" int tla_gram_left_singular(const double* G, int n, int k, const double* W0, int k0, int dtype, void* U,
+                                      void* s, double* U64, double* W_out, double tol, int max_iter, int* iters_out,
+                                      void* ws, size_t ws_bytes, void* stream) {
+  TLA_REQUIRE(dtype == TLA_F32 || dtype == TLA_F64, "gram_left_singular: bad dtype %d", dtype);
+  TLA_REQUIRE(U, "gram_left_singular: NULL U");
+  EigArgs a;
+  int kind = 0;
+  const int rc = eig_args(G, n, k, W0, W0 ? k0 : 0, nullptr, U64, tol, max_iter, iters_out, ws, ws_bytes, &a, &kind);
+  if (rc != TLA_OK) return rc;
+  const int b = tla_gram_svd_block(n, k);
+  a.flip = 1;
+  a.out_dtype = dtype;
+  a.u_out = U;
+  a.s_out = s;
+  a.wblk = b > 0 ? W_out : nullptr;
+  a.wblock = (b > 0 && W0 && k0 == b && a.max_iter >= 2) ? 1 : 0;
   return eig_launch(&a, &kind, 1, (hipStream_t)stream);
 }
 
@@ -1024,6 +1116,7 @@ extern "C" int tla_sym_eig_batch(int count, const double* const* G, const int* n
   EigArgs args[kEigMaxBatch];
   int kinds[kEigMaxBatch];
   for (int i = 0; i < count; ++i) {
+    TLA_REQUIRE(evals[i] && evecs[i], "sym_eig_batch: NULL pointer");
     const int rc = eig_args(G[i], n[i], k[i], Q0 ? Q0[i] : nullptr, k0 ? k0[i] : 0, evals[i], evecs[i], tol,
                             max_iter, nullptr, workspace[i], workspace_bytes[i], &args[i], &kinds[i]);
     if (rc != TLA_OK) return rc;

@@ -18,7 +18,7 @@ import torch
 
 from .. import _lib
 from ..svd import leading_left_singular_vectors, leading_left_singular_vectors_batch
-from ..tenalg import multi_mode_dot, precision as precision_ctx, get_precision
+from ..tenalg import mode_dot, multi_mode_dot, precision as precision_ctx, get_precision
 from ..tucker_tensor import TuckerTensor, validate_tucker_rank
 
 __all__ = ["tucker", "partial_tucker", "initialize_tucker", "Tucker"]
@@ -86,37 +86,45 @@ def partial_tucker(tensor, rank, modes=None, n_iter_max=100, init="svd", tol=10e
     else:
         rank = tuple(rank)
     with precision_ctx(precision or get_precision()):
-        # FP64 copies of the factors warm-start the next eigensolve of their
-        # mode: accurate past float32, a converged HOOI needs one subspace
-        # step per solve
+        # FP64 warm starts for the next eigensolve of each mode: the init's
+        # FP64 vectors, then the solver's whole final block (accurate past
+        # float32, and its guard columns need no re-orthonormalising)
         warm64 = []
         core, factors = initialize_tucker(tensor, rank, modes, init=init, svd=svd, random_state=random_state,
                                           warm64=warm64)
         if len(warm64) != len(modes):
             warm64 = [None] * len(modes)
-        rec_errors = []
-        nx2 = _sqnorm64(tensor)
-        norm_tensor = torch.sqrt(nx2)
-        errs_dev = []
+        nx2 = float(_sqnorm64(tensor))
+        norm_tensor = math.sqrt(nx2)
+
+        def rec_error(c2):  # the reference's expression, in FP64 on the host
+            return math.sqrt(abs(nx2 - c2)) / norm_tensor
+
+        # with the modes in ascending order the last mode's projected tensor
+        # is the core's chain without its last product (multi_mode_dot
+        # applies the modes sorted), so the core is one small product away
+        sorted_modes = list(modes) == sorted(set(modes))
+        core_sq = []  # ||core||^2 per iteration, FP64 device scalars
         for iteration in range(n_iter_max):
             for index, mode in enumerate(modes):
                 y = multi_mode_dot(tensor, factors, modes=modes, skip=index, transpose=True)
-                warm = warm64[index] if warm64[index] is not None else factors[index]
-                U, _, warm64[index] = leading_left_singular_vectors(y, mode, rank[index], warm=warm,
-                                                                    return_fp64=True)
+                U, _, warm64[index] = leading_left_singular_vectors(y, mode, rank[index], warm=warm64[index],
+                                                                    return_warm=True)
                 factors[index] = U
-            core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
-            err = torch.sqrt(torch.abs(nx2 - _sqnorm64(core))) / norm_tensor
-            errs_dev.append(err)
+            if sorted_modes and modes:
+                core = mode_dot(y, factors[-1], modes[-1], transpose=True)
+            else:
+                core = multi_mode_dot(tensor, factors, modes=modes, transpose=True)
+            core_sq.append(_sqnorm64(core))
             if iteration > 1 and (tol or verbose):
-                e1, e2 = float(errs_dev[-2]), float(errs_dev[-1])
+                e1, e2 = rec_error(float(core_sq[-2])), rec_error(float(core_sq[-1]))
                 if verbose:
                     print(f"reconstruction error={e2}, variation={e1 - e2}.")
                 if tol and abs(e1 - e2) < tol:
                     if verbose:
                         print(f"converged in {iteration} iterations.")
                     break
-        rec_errors = [float(e) for e in torch.stack(errs_dev).cpu()] if errs_dev else []
+        rec_errors = [rec_error(float(c2)) for c2 in torch.stack(core_sq).cpu()] if core_sq else []
     return (core, factors), rec_errors
 
 

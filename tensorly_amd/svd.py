@@ -15,7 +15,8 @@ import torch
 from . import _lib
 from ._lib import check, lib, ptr, shape_array, stream, workspace
 
-__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd_non_negative", "gram_eigh", "qr",
+__all__ = ["svd_flip", "svd_checks", "truncated_svd", "svd_interface", "make_svd_non_negative", "gram_eigh",
+           "gram_left_singular", "qr",
            "mode_gram", "SVD_FUNS"]
 
 SVD_FUNS = ["truncated_svd", "symeig_svd"]
@@ -208,19 +209,45 @@ def svd_interface(matrix, method="truncated_svd", n_eigenvecs=None, flip_sign=Tr
     return U, S, V
 
 
-def leading_left_singular_vectors(tensor, mode, k, warm=None, return_fp64=False):
+def gram_left_singular(G, k, dtype, warm=None, want_block=False, tol=1e-11, max_iter=300):
+    """Leading ``k`` left singular vectors (``svd_flip`` signs, ``dtype``) and
+    singular values of the matrix whose FP64 row Gram is ``G``, in one
+    ``tla_gram_left_singular`` launch.  ``warm``: FP64 warm-start columns,
+    or the block an earlier call returned.  Returns ``(U, S, block)``;
+    ``block`` (``want_block``, sizes with a subspace block only) is that
+    call's final FP64 block, the warm start for a nearby matrix."""
+    n = int(G.shape[0])
+    G = G.to(torch.float64).contiguous()
+    b = int(lib().tla_gram_svd_block(n, k))
+    U = torch.empty((n, k), dtype=dtype, device=G.device)
+    S = torch.empty(k, dtype=dtype, device=G.device)
+    q0 = warm.to(torch.float64).contiguous() if (warm is not None and b > 0) else None
+    blk = None
+    if want_block and b > 0:
+        blk = q0 if (q0 is not None and tuple(q0.shape) == (n, b)) else torch.empty((n, b), dtype=torch.float64,
+                                                                                    device=G.device)
+    nb = lib().tla_sym_eig_workspace_size(n)
+    ws = workspace(nb, G.device)
+    check(lib().tla_gram_left_singular(ptr(G), n, k, ptr(q0), q0.shape[1] if q0 is not None else 0,
+                                       _lib.dtype_code(U), ptr(U), ptr(S), None, ptr(blk), float(tol), int(max_iter),
+                                       None, ptr(ws), nb, stream(G.device)), "gram_left_singular")
+    return U, S, blk
+
+
+def leading_left_singular_vectors(tensor, mode, k, warm=None, return_warm=False):
     """Leading ``k`` left singular vectors of ``unfold(tensor, mode)`` with the
     reference sign rule, and the singular values -- without forming the
-    unfolding (the HOOI / init path).  ``return_fp64``: also the FP64
-    vectors (a warm start accurate beyond the tensor's precision)."""
+    unfolding (the HOOI path).  ``return_warm``: also the solver's final
+    FP64 block (None for modes of 128 or fewer rows), the warm start of the
+    next call on the same mode."""
     n = tensor.shape[mode]
     q = tensor.numel() // max(n, 1)
     G = mode_gram(tensor, mode)
-    U, S = _left_svd_from_gram(G, n, min(n, q), k, warm=warm)
-    U, _ = svd_flip(U, None)
-    if return_fp64:
-        return U.to(tensor.dtype), S.to(tensor.dtype), U
-    return U.to(tensor.dtype), S.to(tensor.dtype)
+    U, S, blk = gram_left_singular(G, min(k, n), tensor.dtype, warm=warm, want_block=return_warm)
+    S = S[: min(k, n, q)]
+    if return_warm:
+        return U, S, blk
+    return U, S
 
 
 def leading_left_singular_vectors_batch(tensor, modes, ks):

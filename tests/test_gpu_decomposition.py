@@ -256,6 +256,69 @@ def test_gram_eigensolver_batch_and_warm_start(cuda):
         np.testing.assert_allclose(orth, np.eye(k), atol=1e-11)
 
 
+def _svd_flip_np(U):
+    """The reference's u-based sign rule (tenalg/svd.py:13-42) in numpy."""
+    idx = np.argmax(np.abs(U), axis=0)
+    return U * np.sign(U[idx, np.arange(U.shape[1])])
+
+
+@pytest.mark.parametrize("n,k", [(40, 5), (100, 20), (256, 32), (300, 45), (200, 100)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_gram_left_singular_fused(cuda, n, k, dtype):
+    """tla_gram_left_singular: the leading left singular vectors of a matrix
+    A from its Gram, with svd_flip's signs, against numpy's SVD of A; the
+    same vectors (bit for bit, before the cast) as tla_sym_eig followed by
+    the sign rule on the host; and the returned block warm-starts a solve on
+    a perturbed matrix that lands on the perturbed matrix's own SVD."""
+    from tensorly_amd.svd import gram_eigh, gram_left_singular
+    rng = np.random.default_rng(7 * n + k)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    sv = np.concatenate([np.sort(rng.random(k) + 1.0)[::-1] * 3, rng.random(n - k) * 0.3])
+    A = (Q * sv) @ np.linalg.qr(rng.standard_normal((n + 20, n)))[0].T  # n x (n + 20)
+    G = A @ A.T
+    U, S, blk = gram_left_singular(g(G, cuda), k, dtype, want_block=True)
+    assert U.dtype == dtype and S.dtype == dtype
+    Ur, Sr, _ = np.linalg.svd(A, full_matrices=False)
+    Ur = _svd_flip_np(Ur[:, :k])
+    tol = 1e-6 if dtype == torch.float32 else 1e-9
+    np.testing.assert_allclose(S.cpu().numpy(), Sr[:k], rtol=tol)
+    np.testing.assert_allclose(U.cpu().numpy(), Ur, atol=tol)
+    ev, vec = gram_eigh(g(G, cuda), k)
+    flipped = _svd_flip_np(vec.cpu().numpy())
+    np.testing.assert_array_equal(U.cpu().numpy(), flipped.astype(U.cpu().numpy().dtype))
+    if n <= 128:
+        assert blk is None
+        return
+    assert blk.shape[0] == n and blk.shape[1] >= k + 8
+    np.testing.assert_allclose(blk.cpu().numpy().T @ blk.cpu().numpy(), np.eye(blk.shape[1]), atol=1e-12)
+    A2 = A + 1e-4 * rng.standard_normal(A.shape)
+    G2 = A2 @ A2.T
+    U2, S2, blk2 = gram_left_singular(g(G2, cuda), k, dtype, warm=blk, want_block=True)
+    assert blk2.data_ptr() == blk.data_ptr()  # the block is updated in place
+    Ur2, Sr2, _ = np.linalg.svd(A2, full_matrices=False)
+    np.testing.assert_allclose(S2.cpu().numpy(), Sr2[:k], rtol=tol)
+    np.testing.assert_allclose(U2.cpu().numpy(), _svd_flip_np(Ur2[:, :k]), atol=tol)
+
+
+def test_tucker_core_equals_full_projection(cuda):
+    """HOOI forms each iteration's core from the last mode's projected
+    tensor; with ascending modes that is the same chain of products as the
+    reference's multi_mode_dot over all modes, bit for bit (also for a
+    subset of the modes).  With unsorted modes the reference's skip index
+    (a position in the mode-sorted list) pairs the wrong factors and the
+    core product raises, as it does here."""
+    import tensorly_amd as tla
+    from tensorly_amd.tenalg import multi_mode_dot
+    rng = np.random.default_rng(3)
+    x = g(rng.standard_normal((30, 140, 24)).astype(np.float32), cuda)
+    (core, factors), _ = tla.partial_tucker(x, [5, 12, 6], n_iter_max=3, init="svd", tol=0)
+    assert torch.equal(core, multi_mode_dot(x, factors, modes=[0, 1, 2], transpose=True))
+    (core, factors), _ = tla.partial_tucker(x, [5, 6], modes=[0, 2], n_iter_max=3, init="svd", tol=0)
+    assert torch.equal(core, multi_mode_dot(x, factors, modes=[0, 2], transpose=True))
+    with pytest.raises(ValueError, match="not aligned"):
+        tla.partial_tucker(x, [12, 5], modes=[1, 0], n_iter_max=1, init="svd", tol=0)
+
+
 def test_gram_eigensolver_rejects_large_k(cuda):
     from tensorly_amd.svd import gram_eigh
     G = torch.eye(200, dtype=torch.float64, device=cuda)
