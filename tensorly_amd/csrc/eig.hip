@@ -660,15 +660,31 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_small_kernel(EigBatch bat
 // through a forward elimination of [S | I] held in registers by the block
 // (TPR threads per row; per pivot its row and column go through LDS, one
 // barrier).  S and the factor live in Sbuf (LDS, ld b + 1) and Tbuf (ld ldt).
+//
+// equil (dsc: B doubles of LDS): orthonormalise M D instead, D = diag(S)^-1/2
+// (the same span; columns of very different lengths no longer spoil the
+// conditioning), and report whether one pass suffices: true when every
+// pivot of the equilibrated S is >= 1e-3 (cond(S) ~ 1e3 at most, so the
+// columns come out orthonormal to ~1e-13).  Without equil it returns true.
 template <int B>
-__device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, double* Tbuf, int ldt, bool shift,
-                           double* pbuf, double* dpiv, double* scratch, double* stage) {
+__device__ bool block_orth(const double* M, double* Out, int np, double* Sbuf, double* Tbuf, int ldt, bool shift,
+                           double* pbuf, double* dpiv, double* scratch, double* stage, bool equil = false,
+                           double* dsc = nullptr) {
   constexpr int LDS_S = B + 1, CAP = B;
   // TPR threads per row of [S | I] (rows past B idle), CPT columns each
   constexpr int TPR = B > 64 ? 4 : 8, CPT = 2 * B / TPR;
   static_assert(TPR * B <= kEigThreads && CPT * TPR == 2 * B && TPR % 2 == 0, "thread layout");
   block_gemm_sym_f64(M, CAP, M, CAP, Sbuf, LDS_S, B, np);
   const int tid = threadIdx.x;
+  if (equil) {
+    if (tid < B) dsc[tid] = 1.0 / sqrt(fmax(Sbuf[tid * LDS_S + tid], 1e-300));
+    __syncthreads();
+    for (int e = tid; e < B * B; e += blockDim.x) {
+      const int r = e / B, c = e % B;
+      Sbuf[r * LDS_S + c] *= dsc[r] * dsc[c];
+    }
+    __syncthreads();
+  }
   if (tid == 0) {
     double mx = 0.0;
     for (int i = 0; i < B; ++i) mx = fmax(mx, Sbuf[i * LDS_S + i]);
@@ -707,17 +723,25 @@ __device__ void block_orth(const double* M, double* Out, int np, double* Sbuf, d
       }
     }
   }
+  if (equil && tid == 0) {
+    double mn = INFINITY;
+    for (int k = 0; k < B; ++k) mn = fmin(mn, dpiv[k]);
+    scratch[41] = (mn >= 1e-3) ? 1.0 : 0.0;
+  }
   __syncthreads();
   // x now holds row i of [D L^T | L^-1]; T[r][i] = L^-1[i][r] / sqrt(D[i])
+  // (times dsc[r] when equilibrated)
   const double ds = live ? 1.0 / sqrt(dpiv[i]) : 0.0;
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int c = c0 + j;
-    if (live && c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds;
+    if (live && c >= B) Tbuf[(c - B) * ldt + i] = x[j] * ds * (equil ? dsc[c - B] : 1.0);
   }
+  const bool well = !equil || scratch[41] != 0.0;
   __syncthreads();
   if constexpr (B == 48) block_gemm_f64_panel<B>(M, Tbuf, ldt, Out, CAP, np, B, stage);
   else block_gemm_f64<false>(M, CAP, Tbuf, ldt, Out, CAP, np, B, B);
+  return well;
 }
 
 // Block subspace iteration with Rayleigh-Ritz for n > kJacLarge.  B is the
@@ -780,8 +804,12 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // next Rayleigh-Ritz step rotates to the final threshold
     if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
     else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
-    block_orth<B>(Z, U, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
-    block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    // G V's columns are close to multiples of the warm columns: once
+    // equilibrated, one unshifted Cholesky-QR pass usually suffices
+    if (!block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage, true, resid)) {
+      block_orth<B>(Z, U, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+      block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    }
     jthr = jfloor;
     it = 1;
   }
@@ -852,10 +880,13 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     jthr = scratch[62];
     __syncthreads();
     if (done || it + 1 == a.max_iter) break;
-    // V <- orth(U), two Cholesky-QR passes (the columns of U = G Y scale
-    // with the Ritz values)
-    block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
-    block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    // V <- orth(U): the columns of U = G Y scale with the Ritz values, so
+    // equilibrated first, one pass when that leaves S well conditioned,
+    // else two Cholesky-QR passes (shifted, then plain)
+    if (!block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage, true, resid)) {
+      block_orth<B>(U, Z, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+      block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    }
   }
   eig_emit(a, Y, CAP, nullptr, theta, resid, B);
   if (tid == 0 && a.iters) *a.iters = it + 1;
