@@ -744,11 +744,50 @@ __device__ bool block_orth(const double* M, double* Out, int np, double* Sbuf, d
   return well;
 }
 
+// Z = G V of the warm-block path spread over the GPU: one wave per 16 x 16
+// tile of Z (blockIdx.x: tile, blockIdx.y: problem), V = Y (the warm block,
+// src 0) or V (src 1).  G read as G^T, the same k-ascending MFMA sequence per
+// tile as gv_product, so Z is the same to the bit.
+template <int B>
+__global__ __launch_bounds__(64) void eig_gv_kernel(EigBatch batch, int src) {
+  const EigArgs a = batch.a[blockIdx.y];
+  constexpr int TN = B / 16;
+  const int np = a.np;
+  if ((int)blockIdx.x >= (np / 16) * TN) return;
+  const int mi = (blockIdx.x / TN) * 16, nj = (blockIdx.x % TN) * 16;
+  const double* V = src ? a.V : a.Y;
+  const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+  const double* gp = a.Gp + mi + lr;
+  doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+  int k0 = 0;
+  for (; k0 + 64 <= np; k0 += 64) {
+    double x[16], y[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int kk = k0 + 4 * s + lk;
+      x[s] = gp[(int64_t)kk * np];
+      y[s] = V[kk * B + nj + lr];
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) acc = mfma_f64(x[s], y[s], acc);
+  }
+  for (; k0 < np; k0 += 4) {
+    const int kk = k0 + lk;
+    acc = mfma_f64(gp[(int64_t)kk * np], V[kk * B + nj + lr], acc);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) a.Z[(int64_t)(mi + lk + 4 * e) * B + nj + lr] = acc[e];
+}
+
 // Block subspace iteration with Rayleigh-Ritz for n > kJacLarge.  B is the
 // block: 48 and 64 keep H, W and T in LDS; 128 keeps H in LDS and W, T in the
 // workspace.  All products are FP64 MFMA on zero-padded np-row blocks.
+// phase 0: the whole solve.  A warm block (a.wblock) instead runs as
+// eig_gv_kernel (Z = G Y), phase 1 (V <- orth(Z)), eig_gv_kernel (Z = G V),
+// phase 2 (the solve from its first Rayleigh-Ritz step), so that both
+// products spread over the GPU.
 template <int B>
-__global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch batch) {
+__global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch batch, int phase) {
   const EigArgs a = batch.a[blockIdx.x];
   constexpr int LDH = B + 1, CAP = B;
   constexpr bool SMALL = B <= 64;
@@ -776,7 +815,22 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
   double* U = a.U;
   // Gp (zero-padded symmetric G) and the starting block Y come from
   // eig_prep_kernel, launched just before on the same stream
-  if (a.Q0 && a.k0 > 0 && a.wblock) {
+  if (phase == 1) {
+    if (!block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage, true, resid)) {
+      block_orth<B>(Z, U, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+      block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+    }
+    return;
+  }
+  const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
+  double jthr = 1e-10;
+  int it = 0;
+  bool have_z = false;  // phase 2: Z = G V is there already
+  if (phase == 2) {
+    jthr = jfloor;
+    it = 1;
+    have_z = true;
+  } else if (a.Q0 && a.k0 > 0 && a.wblock) {
     // a whole block from an earlier solve: the plain subspace step below
     // orthonormalises G V, so V itself is used as it is
     double* t = V;
@@ -794,10 +848,7 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     V = Y;
     Y = t;
   }
-  const double jfloor = fmax(1e-30, 1e-2 * a.tol * a.tol);
-  double jthr = 1e-10;
-  int it = 0;
-  if (a.Q0 && a.k0 > 0 && a.max_iter >= 2) {
+  if (phase == 0 && a.Q0 && a.k0 > 0 && a.max_iter >= 2) {
     // warm start: its first Ritz test fails whenever the matrix has moved by
     // more than the tolerance, so the first step is a plain subspace step,
     // V <- orth(G V) (the span a Rayleigh-Ritz step would leave), and the
@@ -817,8 +868,11 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // Z = G V, read as G^T V (Gp is exactly symmetric): each load then takes
     // 4 rows x 128 contiguous bytes of G instead of 16 rows x 32 bytes, which
     // measured 2.3x faster
-    if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
-    else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
+    if (!have_z) {
+      if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
+      else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
+    }
+    have_z = false;
     block_gemm_sym_f64(V, CAP, Z, CAP, H, LDH, B, np);           // H = V^T Z = V^T G V
     for (int e = tid; e < B * B; e += nt) {
       const int i = e / B, j = e % B;
@@ -1068,6 +1122,18 @@ static int eig_args(const double* G, int n, int k, const double* Q0, int k0, dou
   return TLA_OK;
 }
 
+// The warm-block sequence of sym_eig_subspace_kernel (m problems).
+template <int B>
+static void launch_split(const EigBatch& b, int m, hipStream_t st) {
+  int tiles = 0;
+  for (int i = 0; i < m; ++i) tiles = std::max(tiles, (b.a[i].np / 16) * (B / 16));
+  const dim3 g((unsigned)tiles, (unsigned)m);
+  eig_gv_kernel<B><<<g, 64, 0, st>>>(b, 0);
+  sym_eig_subspace_kernel<B><<<m, kEigThreads, 0, st>>>(b, 1);
+  eig_gv_kernel<B><<<g, 64, 0, st>>>(b, 1);
+  sym_eig_subspace_kernel<B><<<m, kEigThreads, 0, st>>>(b, 2);
+}
+
 // One launch per kernel kind present, one workgroup per problem.
 static int eig_launch(const EigArgs* args, const int* kinds, int count, hipStream_t st) {
   for (int kind = 0; kind < 5; ++kind) {
@@ -1092,12 +1158,14 @@ static int eig_launch(const EigArgs* args, const int* kinds, int count, hipStrea
       }
       TLA_LAUNCH_CHECK();
     }
+    bool split = kind >= 2;  // every problem a warm block
+    for (int i = 0; i < m; ++i) split = split && b.a[i].wblock;
     switch (kind) {
       case 0: sym_eig_small_kernel<kJac><<<m, kEigThreads, 0, st>>>(b); break;
       case 1: sym_eig_small_kernel<kJacLarge><<<m, kEigThreads, 0, st>>>(b); break;
-      case 2: sym_eig_subspace_kernel<48><<<m, kEigThreads, 0, st>>>(b); break;
-      case 3: sym_eig_subspace_kernel<64><<<m, kEigThreads, 0, st>>>(b); break;
-      default: sym_eig_subspace_kernel<kEigCapLarge><<<m, kEigThreads, 0, st>>>(b); break;
+      case 2: split ? launch_split<48>(b, m, st) : sym_eig_subspace_kernel<48><<<m, kEigThreads, 0, st>>>(b, 0); break;
+      case 3: split ? launch_split<64>(b, m, st) : sym_eig_subspace_kernel<64><<<m, kEigThreads, 0, st>>>(b, 0); break;
+      default: sym_eig_subspace_kernel<kEigCapLarge><<<m, kEigThreads, 0, st>>>(b, 0); break;
     }
     TLA_LAUNCH_CHECK();
   }
