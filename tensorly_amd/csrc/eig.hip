@@ -966,21 +966,34 @@ __global__ __launch_bounds__(256) void mode_gram_kernel(const T* __restrict__ X,
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int mi = (wave >> 1) * 16, nj = (wave & 1) * 16;
   doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+  // thread: column qq of the 64-wide chunk, rows r0, r0 + 4, ..., r0 + 28 of
+  // both 32-row blocks; the chunk after the current one is loaded into
+  // registers while the current one is multiplied
+  const int qq = tid & 63, r0 = tid >> 6;
+  T ci[8], cj[8];
+  auto fetch = [&](int64_t qb) {
+    const int64_t q = qb + qq;
+    int64_t base = -1;
+    if (q < q1) {
+      const int64_t aa = q / Pb;
+      base = aa * In * Pb + (q - aa * Pb);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t gi = (int64_t)ti * 32 + r0 + 4 * u, gj = (int64_t)tj * 32 + r0 + 4 * u;
+      ci[u] = (base >= 0 && gi < In) ? X[base + gi * Pb] : (T)0;
+      cj[u] = (base >= 0 && gj < In) ? X[base + gj * Pb] : (T)0;
+    }
+  };
+  if (q0 < q1) fetch(q0);
   for (int64_t qb = q0; qb < q1; qb += 64) {
-    for (int e = tid; e < 32 * 64; e += 256) {
-      const int r = e / 64, qq = e % 64;
-      const int64_t q = qb + qq;
-      const int64_t gi = (int64_t)ti * 32 + r, gj = (int64_t)tj * 32 + r;
-      double vi = 0.0, vj = 0.0;
-      if (q < q1) {
-        const int64_t aa = q / Pb, bb = q - aa * Pb;
-        if (gi < In) vi = (double)X[(aa * In + gi) * Pb + bb];
-        if (gj < In) vj = (double)X[(aa * In + gj) * Pb + bb];
-      }
-      Ti[r][qq] = vi;
-      Tj[r][qq] = vj;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      Ti[r0 + 4 * u][qq] = (double)ci[u];
+      Tj[r0 + 4 * u][qq] = (double)cj[u];
     }
     __syncthreads();
+    if (qb + 64 < q1) fetch(qb + 64);
     // wave w: the 16 x 16 output tile (w / 2, w % 2) of the 32 x 32 block on
     // FP64 MFMA (products of the float inputs are exact in FP64)
 #pragma unroll
