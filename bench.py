@@ -250,8 +250,9 @@ def precision_study(torch, dev):
     against the FP64 numpy oracle for FP32, BF16 and bf16x3 (with and
     without power-of-two column scaling), per-MTTKRP kernel time; then 200
     ALS sweeps in each precision from the reference's SVD init (next to an
-    FP64 GPU run) and from a random init, with the first non-finite sweep if
-    any."""
+    FP64 GPU run and the FP64 numpy oracle's run of tests/golden/
+    c3_oracle_fit.npz) and from a random init, with the first non-finite
+    sweep if any."""
     import tensorly_amd as tla
     from oracle import tensorly_np as T
     from tensorly_amd import synthetic
@@ -332,7 +333,7 @@ def precision_study(torch, dev):
     log(f"[study] CPU MTTKRP: {cpu_row}")
     ones = torch.ones(R, dtype=torch.float32, device=dev)
 
-    def als(prec, f0, n, x=X, w=ones, normalize=False):
+    def als(prec, f0, n, x=X, w=ones, normalize=False, compare=False):
         eng = CpAlsEngine(x, w, f0, precision=prec, dimtree=(prec in ("bf16", "bf16x3")),
                           tensor_bf16=(Xb if prec == "bf16" else None))
         errs = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -345,9 +346,20 @@ def precision_study(torch, dev):
         e = errs.cpu().numpy()
         bad = np.flatnonzero(~np.isfinite(e))
         marks = [s for s in (0, 9, 24, 49, 99, 149, 199) if s < n]
-        return {"final_error": float(e[-1]), "fit": float(1 - e[-1]), "ms_per_sweep": round(dt / n * 1e3, 4),
-                "first_nonfinite_sweep": int(bad[0]) if len(bad) else None,
-                "error_at_sweep": {str(s + 1): float(e[s]) for s in marks}}
+        res = {"final_error": float(e[-1]), "fit": float(1 - e[-1]), "ms_per_sweep": round(dt / n * 1e3, 4),
+               "first_nonfinite_sweep": int(bad[0]) if len(bad) else None,
+               "error_at_sweep": {str(s + 1): float(e[s]) for s in marks}}
+        if compare and oracle_errs is not None:
+            m = min(n, len(oracle_errs))
+            res["max_abs_error_diff_vs_oracle"] = {"sweeps": m, "value": float(np.max(np.abs(e[:m] - oracle_errs[:m])))}
+        return res
+
+    # the FP64 numpy oracle's run from init='svd' (written by
+    # tests/golden/make_c3_oracle_fit.py; absent: not compared)
+    try:
+        oracle_errs = np.load(os.path.join(ROOT, "tests", "golden", "c3_oracle_fit.npz"))["errors"]
+    except OSError:
+        oracle_errs = None
 
     # (1) the reference's default init='svd' (computed once in FP64, cast):
     # every precision starts from the same point
@@ -355,15 +367,20 @@ def precision_study(torch, dev):
     _, f_svd = tla.decomposition.initialize_cp(X64, R, init="svd")
     fits = {}
     for prec in ("fp32", "bf16", "bf16x3"):
-        fits[prec] = als(prec, [f.float() for f in f_svd], 200)
+        fits[prec] = als(prec, [f.float() for f in f_svd], 200, compare=True)
         log(f"[study] ALS svd init {prec}: {fits[prec]}")
     # BF16 with the factors renormalised after every sweep (weights carry the scale)
-    fits["bf16_normalized"] = als("bf16", [f.float() for f in f_svd], 200, normalize=True)
+    fits["bf16_normalized"] = als("bf16", [f.float() for f in f_svd], 200, normalize=True, compare=True)
     log(f"[study] ALS svd init bf16 normalised: {fits['bf16_normalized']}")
-    fits["fp64"] = als("valu", f_svd, 200, x=X64, w=ones.double())
+    fits["fp64"] = als("valu", f_svd, 200, x=X64, w=ones.double(), compare=True)
     log(f"[study] ALS svd init fp64: {fits['fp64']}")
     for prec in ("fp32", "bf16", "bf16x3", "bf16_normalized"):
         fits[prec]["fit_minus_fp64"] = fits[prec]["fit"] - fits["fp64"]["fit"]
+    if oracle_errs is not None and len(oracle_errs) >= 200:
+        fits["fp64_oracle"] = {"fit": float(1 - oracle_errs[199]), "final_error": float(oracle_errs[199]),
+                               "source": "tests/golden/c3_oracle_fit.npz (numpy FP64 restatement, init='svd')"}
+        for prec in ("fp32", "bf16", "bf16x3", "bf16_normalized", "fp64"):
+            fits[prec]["fit_minus_oracle"] = fits[prec]["fit"] - fits["fp64_oracle"]["fit"]
     out["als_svd_init_200_sweeps"] = fits
     del X64
     # (2) init='random' (RandomState(2)): reports where a run stops being finite
