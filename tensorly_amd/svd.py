@@ -251,9 +251,10 @@ def leading_left_singular_vectors(tensor, mode, k, warm=None, return_warm=False)
 
 
 def leading_left_singular_vectors_batch(tensor, modes, ks):
-    """:func:`leading_left_singular_vectors` (with ``return_fp64``) of several
-    modes of one tensor, cold-started, the eigensolves in one batched launch
-    (the Tucker SVD init): a list of ``(U, S, U64)``."""
+    """:func:`leading_left_singular_vectors` of several modes of one tensor,
+    cold-started, the eigensolves in one batched launch (the Tucker SVD
+    init): a list of ``(U, S, U64)``, U64 the FP64 vectors (the first warm
+    start of each mode's HOOI solves)."""
     Gs, kk, dims = [], [], []
     for mode, k in zip(modes, ks):
         n = tensor.shape[mode]
