@@ -314,6 +314,121 @@ __device__ __forceinline__ void jac_apply_w(double* W, const double2* cr, const 
   }
 }
 
+// True when the `lead` largest eigenvalues of the symmetric A (m x m, LDS)
+// belong to its leading rows: the Gershgorin discs of rows < lead lie above
+// those of the other rows, and the coupling between the two sets is at most
+// rel_off2 of ||A||_F^2 (uniform across the block; scratch[0..34]).
+template <int LDA>
+__device__ bool lead_block_separated(const double* A, int m, int lead, double rel_off2, double* scratch) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // per row: Gershgorin radius, squared coupling to the other set, squares
+  double lo = INFINITY, hi = -INFINITY, cpl = 0.0, tot = 0.0;
+  if (tid < m) {
+    double rad = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double v = A[tid * LDA + j];
+      tot += v * v;
+      if (j != tid) rad += fabs(v);
+      if ((j < lead) != (tid < lead)) cpl += v * v;
+    }
+    const double d = A[tid * LDA + tid];
+    if (tid < lead) lo = d - rad;
+    else hi = d + rad;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off, 64));
+    hi = fmax(hi, __shfl_xor(hi, off, 64));
+  }
+  cpl = wave_sum(cpl);
+  tot = wave_sum(tot);
+  if ((tid & 63) == 0) {
+    const int w = tid >> 6;
+    scratch[w] = lo; scratch[8 + w] = hi; scratch[16 + w] = cpl; scratch[24 + w] = tot;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double l = INFINITY, h = -INFINITY, c = 0.0, t = 0.0;
+    for (int w = 0; w < nt / 64 && w < 8; ++w) {
+      l = fmin(l, scratch[w]); h = fmax(h, scratch[8 + w]); c += scratch[16 + w]; t += scratch[24 + w];
+    }
+    scratch[34] = (l > h && c <= rel_off2 * t) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const bool ok = scratch[34] != 0.0;
+  __syncthreads();
+  return ok;
+}
+
+// Eigenvectors of the leading ma x ma block of a symmetric A that is nearly
+// diagonal, A = D + E (D its diagonal), by second-order perturbation theory
+// instead of Jacobi sweeps: with M[k][i] = E[k][i] / (d_i - d_k) (k != i),
+//   v_i = e_i + M e_i + sum_k e_k (E M)[k][i] / (d_i - d_k) - e_i sum_k M[k][i]^2 / 2,
+//   lambda_i = d_i + sum_k E[k][i] M[k][i],
+// correct to third order in max |M|.  Taken only when max |M| <= 1e-5 (no
+// close eigenvalues relative to the coupling): the neglected terms, sums of
+// up to ma^2 products of three entries of M, stay ~1e-12 of the vectors;
+// else false and nothing is written.  On success W[0:ma, 0:ma] = the vectors
+// (W is the identity on entry) and A's leading diagonal = the eigenvalues
+// (its off-diagonal entries are left as they were).  Mbuf, Pbuf: ma x ma
+// LDS scratch (stride ma).
+template <int LDA, int LDW>
+__device__ bool perturb_eigvecs(double* A, double* W, int ma, double* Mbuf, double* Pbuf, double* scratch) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double mmax = 0.0;
+  for (int e = tid; e < ma * ma; e += nt) {
+    const int k = e / ma, i = e % ma;
+    double mv = 0.0;
+    if (k != i) mv = A[k * LDA + i] / (A[i * LDA + i] - A[k * LDA + k]);
+    Mbuf[e] = mv;
+    mmax = fmax(mmax, isfinite(mv) ? fabs(mv) : INFINITY);
+  }
+  for (int off = 32; off > 0; off >>= 1) mmax = fmax(mmax, __shfl_xor(mmax, off, 64));
+  if ((tid & 63) == 0) scratch[tid >> 6] = mmax;
+  __syncthreads();
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int w = 0; w < nt / 64 && w < 8; ++w) mx = fmax(mx, scratch[w]);
+    scratch[35] = (mx <= 1e-5) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (scratch[35] == 0.0) {
+    __syncthreads();
+    return false;
+  }
+  // P = E M (E: A without its diagonal)
+  for (int e = tid; e < ma * ma; e += nt) {
+    const int k = e / ma, i = e % ma;
+    double pv = 0.0;
+    for (int l = 0; l < ma; ++l)
+      if (l != k) pv = fma(A[k * LDA + l], Mbuf[l * ma + i], pv);
+    Pbuf[e] = pv;
+  }
+  __syncthreads();
+  for (int e = tid; e < ma * ma; e += nt) {
+    const int k = e / ma, i = e % ma;
+    double wv;
+    if (k != i) {
+      wv = Mbuf[e] + Pbuf[e] / (A[i * LDA + i] - A[k * LDA + k]);
+    } else {
+      double n2 = 0.0;
+      for (int l = 0; l < ma; ++l) n2 = fma(Mbuf[l * ma + i], Mbuf[l * ma + i], n2);
+      wv = 1.0 - 0.5 * n2;
+    }
+    W[k * LDW + i] = wv;
+  }
+  // eigenvalues after every thread has read the diagonal above
+  double lam = 0.0;
+  if (tid < ma) {
+    lam = A[tid * LDA + tid];
+    for (int k = 0; k < ma; ++k)
+      if (k != tid) lam = fma(A[k * LDA + tid], Mbuf[k * ma + tid], lam);
+  }
+  __syncthreads();
+  if (tid < ma) A[tid * LDA + tid] = lam;
+  __syncthreads();
+  return true;
+}
+
 // One sweep is me - 1 rounds of me / 2 disjoint rotations.  Per round: wave 0
 // computes the round's rotations (c, s and the pair p | q << 16 per slot, in
 // cs / pr, two rounds' worth each) while the other waves apply the previous
@@ -324,51 +439,18 @@ __device__ __forceinline__ void jac_apply_w(double* W, const double2* cr, const 
 // lead > 0: only the eigenpairs of the `lead` largest eigenvalues are wanted.
 // If the Gershgorin discs of rows < lead lie above those of the other rows
 // (so the lead largest eigenvalues belong to the leading rows) and the
-// coupling between the two sets is below the threshold, the sweeps rotate
-// the leading lead x lead block only; the coupling and the trailing block are
-// left as they are (W must be the identity on entry; its trailing columns
-// stay so).  Returns the number of sweeps.
+// coupling between the two sets is below the threshold
+// (lead_block_separated; lead_known: its answer if already known, -1 not),
+// the sweeps rotate the leading lead x lead block only; the coupling and the
+// trailing block are left as they are (W must be the identity on entry; its
+// trailing columns stay so).  Returns the number of sweeps.
 template <int LDA, int LDW>
 __device__ int lds_jacobi(double* A, double* W, int m, int max_sweeps, double2* cs, int* pr, double* scratch,
-                          double rel_off2 = 1e-30, int lead = 0) {
+                          double rel_off2 = 1e-30, int lead = 0, int lead_known = -1) {
   const int tid = threadIdx.x, nt = blockDim.x;
   int ma = m;  // the rows and columns the sweeps rotate
-  if (lead > 0 && lead < m) {
-    // per row: Gershgorin radius, squared coupling to the other set, squares
-    double lo = INFINITY, hi = -INFINITY, cpl = 0.0, tot = 0.0;
-    if (tid < m) {
-      double rad = 0.0;
-      for (int j = 0; j < m; ++j) {
-        const double v = A[tid * LDA + j];
-        tot += v * v;
-        if (j != tid) rad += fabs(v);
-        if ((j < lead) != (tid < lead)) cpl += v * v;
-      }
-      const double d = A[tid * LDA + tid];
-      if (tid < lead) lo = d - rad;
-      else hi = d + rad;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      lo = fmin(lo, __shfl_xor(lo, off, 64));
-      hi = fmax(hi, __shfl_xor(hi, off, 64));
-    }
-    cpl = wave_sum(cpl);
-    tot = wave_sum(tot);
-    if ((tid & 63) == 0) {
-      const int w = tid >> 6;
-      scratch[w] = lo; scratch[8 + w] = hi; scratch[16 + w] = cpl; scratch[24 + w] = tot;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double l = INFINITY, h = -INFINITY, c = 0.0, t = 0.0;
-      for (int w = 0; w < nt / 64 && w < 8; ++w) {
-        l = fmin(l, scratch[w]); h = fmax(h, scratch[8 + w]); c += scratch[16 + w]; t += scratch[24 + w];
-      }
-      scratch[34] = (l > h && c <= rel_off2 * t) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    if (scratch[34] != 0.0) ma = lead;
-  }
+  if (lead > 0 && lead < m && (lead_known == 1 || (lead_known < 0 && lead_block_separated<LDA>(A, m, lead, rel_off2, scratch))))
+    ma = lead;
   const int me = ma + (ma & 1);  // even number of slots (one dummy)
   const int npairs = me / 2, nblk = npairs * npairs;
   const JacIter blk0(tid, nt, npairs);                         // A blocks (pa, pb)
@@ -883,7 +965,17 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     // the Ritz rotations only need the accuracy of the current subspace:
     // off-diagonal mass down to (residual / t_1)^2 / 100 of the total,
     // tightening to tol^2 / 100 as the residuals fall
-    lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch, jthr, k);
+    {
+      // a nearly diagonal leading block (a converging warm start): second-
+      // order perturbation instead of Jacobi sweeps when it is safe (block
+      // 48: its two k x k scratch matrices use the panel staging area)
+      const bool sep = k < B && lead_block_separated<LDH>(H, B, k, jthr, scratch);
+      bool pert = false;
+      if constexpr (B == 48) {
+        if (sep) pert = perturb_eigvecs<LDH, LDW>(H, W, k, stage, stage + k * k, scratch);
+      }
+      if (!pert) lds_jacobi<LDH, LDW>(H, W, B, 30, cs, pr, scratch, jthr, k, sep ? 1 : 0);
+    }
     sort_desc<LDH>(H, B, order);
     if (tid < B) theta[tid] = H[order[tid] * LDH + order[tid]];
     __syncthreads();
