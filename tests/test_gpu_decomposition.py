@@ -268,8 +268,8 @@ def test_gram_left_singular_fused(cuda, n, k, dtype):
     """tla_gram_left_singular: the leading left singular vectors of a matrix
     A from its Gram, with svd_flip's signs, against numpy's SVD of A; the
     same vectors (bit for bit, before the cast) as tla_sym_eig followed by
-    the sign rule on the host; and the returned block warm-starts a solve on
-    a perturbed matrix that lands on the perturbed matrix's own SVD."""
+    the sign rule on the host; and the returned block warm-starts solves on
+    perturbed matrices (1e-4, then 1e-8) that land on their own SVDs."""
     from tensorly_amd.svd import gram_eigh, gram_left_singular
     rng = np.random.default_rng(7 * n + k)
     Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
@@ -298,6 +298,13 @@ def test_gram_left_singular_fused(cuda, n, k, dtype):
     Ur2, Sr2, _ = np.linalg.svd(A2, full_matrices=False)
     np.testing.assert_allclose(S2.cpu().numpy(), Sr2[:k], rtol=tol)
     np.testing.assert_allclose(U2.cpu().numpy(), _svd_flip_np(Ur2[:, :k]), atol=tol)
+    # a much smaller move (the leading Ritz block is then diagonal to ~1e-8,
+    # the case its perturbation-theory eigenvectors take)
+    A3 = A2 + 1e-8 * rng.standard_normal(A.shape)
+    U3, S3, _ = gram_left_singular(g(A3 @ A3.T, cuda), k, dtype, warm=blk2, want_block=True)
+    Ur3, Sr3, _ = np.linalg.svd(A3, full_matrices=False)
+    np.testing.assert_allclose(S3.cpu().numpy(), Sr3[:k], rtol=tol)
+    np.testing.assert_allclose(U3.cpu().numpy(), _svd_flip_np(Ur3[:, :k]), atol=tol)
 
 
 def test_tucker_core_equals_full_projection(cuda):
