@@ -104,13 +104,25 @@ def partial_tucker(tensor, rank, modes=None, n_iter_max=100, init="svd", tol=10e
         # is the core's chain without its last product (multi_mode_dot
         # applies the modes sorted), so the core is one small product away
         sorted_modes = list(modes) == sorted(set(modes))
+        # and the products of the tensor with the first index modes' new
+        # factors (prefix) are shared by every later mode of the iteration:
+        # y_index = prefix_index x (the later modes), the same sequence of
+        # products as multi_mode_dot(..., skip=index)
         core_sq = []  # ||core||^2 per iteration, FP64 device scalars
         for iteration in range(n_iter_max):
+            prefix = tensor
             for index, mode in enumerate(modes):
-                y = multi_mode_dot(tensor, factors, modes=modes, skip=index, transpose=True)
+                if sorted_modes:
+                    y = prefix
+                    for later in range(index + 1, len(modes)):
+                        y = mode_dot(y, factors[later], modes[later], transpose=True)
+                else:
+                    y = multi_mode_dot(tensor, factors, modes=modes, skip=index, transpose=True)
                 U, _, warm64[index] = leading_left_singular_vectors(y, mode, rank[index], warm=warm64[index],
                                                                     return_warm=True)
                 factors[index] = U
+                if sorted_modes and index + 1 < len(modes):
+                    prefix = mode_dot(prefix, U, mode, transpose=True)
             if sorted_modes and modes:
                 core = mode_dot(y, factors[-1], modes[-1], transpose=True)
             else:
