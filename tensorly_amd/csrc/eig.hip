@@ -945,6 +945,18 @@ __global__ __launch_bounds__(kEigThreads) void sym_eig_subspace_kernel(EigBatch 
     }
     jthr = jfloor;
     it = 1;
+  } else if (phase == 0 && !(a.Q0 && a.k0 > 0)) {
+    // cold start: two plain subspace steps V <- orth(G V) before the first
+    // Rayleigh-Ritz step (whose Ritz matrix on a random block is dense and
+    // costs full Jacobi sweeps); a plain step is a fraction of a Ritz step
+    for (int p = 0; p < 2 && it + 2 < a.max_iter; ++p, ++it) {
+      if constexpr (B == 48) gv_product<B, 128>(a.Gp, np, V, Z, stage);
+      else block_gemm_f64<true>(a.Gp, np, V, CAP, Z, CAP, np, B, np);
+      if (!block_orth<B>(Z, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage, true, resid)) {
+        block_orth<B>(Z, U, np, H, T, ldt, true, pbuf, dpiv, scratch, stage);
+        block_orth<B>(U, V, np, H, T, ldt, false, pbuf, dpiv, scratch, stage);
+      }
+    }
   }
   for (; it < a.max_iter; ++it) {
     // Z = G V, read as G^T V (Gp is exactly symmetric): each load then takes
