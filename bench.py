@@ -542,7 +542,7 @@ def batched_report(torch, dev, net, log):
                      "failed": int((out["status"].cpu().numpy() != 0).sum()),
                      "mean_final_error": float(out["error"].cpu().numpy().mean()),
                      "extrapolations": ne, "fallback_rate": (1.0 - na / ne) if ne else None,
-                     "launch": dict(b.config), "autotune_ms": {f"{t['waves']}x{t['blocks_per_cu']}": round(t["ms"], 1)
+                     "launch": dict(b.config), "autotune_ms": {f"{t['waves']}x{t['blocks_per_cu']}x{t['ktile']}": round(t["ms"], 1)
                                                                for t in tt}}
         sw_all = out["sweeps"].cpu().numpy()
         hist[name] = (out["hist_err"].cpu().numpy(), sw_all, out["error"].cpu().numpy())

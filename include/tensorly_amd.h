@@ -493,7 +493,10 @@ int tla_extrap_features(const double* errors, const float* alphas, const int* ac
  * for policies 0 and 1) are [n_problems][max_sweeps].  Outputs per problem:
  * sweeps, tensor passes, final relative error, extrapolations tried and
  * kept, status (0 ok, 2 singular, 3 non-finite), ticks (nullable, 100 MHz
- * wall clock).  waves: 2, 4 or 8 per workgroup; grid: workgroups.
+ * wall clock).  waves: 2, 4 or 8 per workgroup; grid: workgroups; ktile:
+ * the MTTKRP's reduction tile, 16, 32 or 64 reduction indices in flight
+ * per wave (the same results to the bit; a launch-shape parameter for the
+ * autotuner).
  * Needs no workspace beyond the arrays above and allocates nothing, so the
  * call can be captured in a HIP graph (the queue reset is a kernel).
  * tla_batched_cp_max_blocks_per_cu: resident workgroups per CU for `waves`.
@@ -506,7 +509,7 @@ int tla_batched_cp_als(int n_problems, const float* X, const int64_t* x_off, con
                        int max_decisions, int first_decision, double* hist_err, float* hist_alpha, int* hist_acc, float* features,
                        double* probe_err, int* sweeps_out, int* passes_out, double* err_out, int* n_extrap_out,
                        int* n_accept_out, int* status_out, long long* ticks_out, int waves, int grid,
-                       void* stream);
+                       int ktile, void* stream);
 
 #ifdef __cplusplus
 }

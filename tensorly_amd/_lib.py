@@ -100,7 +100,7 @@ _SIGNATURES = {
     "tla_batched_cp_max_blocks_per_cu": (c_i, [c_i]),
     "tla_batched_cp_als": (c_i, [c_i, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i, c_i, c_d, c_i, c_p, c_f,
                                  c_f, c_p, c_i, c_i, c_i, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i,
-                                 c_i, c_p]),
+                                 c_i, c_i, c_p]),
 }
 
 

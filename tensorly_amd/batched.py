@@ -166,7 +166,7 @@ class BatchedCP:
 
     def launch(self, factors, out, policy="plain", max_sweeps=1000, tol=1e-7, warmup=3, model=None,
                alpha_fixed=0.0, alpha_max=16.0, probe_alphas=None, max_decisions=0, first_decision=0, waves=None,
-               grid=None):
+               grid=None, ktile=None):
         """Enqueue one batched run on the current stream (no host sync).
         ``factors`` (packed, float32) is updated in place; ``out`` comes from
         :meth:`make_outputs`.  Safe to capture in a graph."""
@@ -177,10 +177,11 @@ class BatchedCP:
             raise ValueError("policy 'probe' needs probe_alphas and make_outputs(..., max_decisions=, n_probe=)")
         if factors.dtype != torch.float32 or factors.device != self.device or factors.numel() < self.n_factor:
             raise ValueError("factors must be the packed float32 factor vector of this batch")
+        cfg = self.config or {"waves": 4, "blocks_per_cu": None, "ktile": 32}
         if waves is None or grid is None:
-            cfg = self.config or {"waves": 4, "blocks_per_cu": None}
             waves = waves or cfg["waves"]
             grid = grid or self._grid(waves, cfg.get("blocks_per_cu"))
+        ktile = int(ktile or cfg.get("ktile", 32))
         params = model.params if model is not None else None
         n_probe = 0 if probe_alphas is None else int(probe_alphas.numel())
         check(lib().tla_batched_cp_als(
@@ -190,7 +191,7 @@ class BatchedCP:
             ptr(probe_alphas), n_probe, int(max_decisions), int(first_decision), ptr(out.get("hist_err")), ptr(out.get("hist_alpha")),
             ptr(out.get("hist_acc")), ptr(out.get("features")), ptr(out.get("probe_err")), ptr(out["sweeps"]),
             ptr(out["passes"]), ptr(out["error"]), ptr(out["n_extrap"]), ptr(out["n_accept"]), ptr(out["status"]),
-            ptr(out["ticks"]), int(waves), int(grid), _lib.stream(self.device)), "batched_cp_als")
+            ptr(out["ticks"]), int(waves), int(grid), ktile, _lib.stream(self.device)), "batched_cp_als")
 
     def run(self, factors=None, seed=0, policy="plain", max_sweeps=1000, tol=1e-7, history=False, **kw):
         """Run to convergence; returns (packed factors, outputs dict)."""
@@ -224,7 +225,8 @@ class BatchedCP:
     def autotune(self, candidates=None, sweeps=6, seed=0, repeats=2, converge=False, policy="plain",
                  max_sweeps=1000, tol=1e-7, **policy_kw):
         """Time candidate launch shapes (waves per workgroup x resident
-        workgroups per CU) and keep the fastest.  Returns the timing table.
+        workgroups per CU x the MTTKRP's reduction tile of 16, 32 or 64
+        indices per wave) and keep the fastest.  Returns the timing table.
 
         ``converge=False`` times ``sweeps`` plain sweeps of every problem: a
         throughput measure.  ``converge=True`` times whole runs of ``policy``
@@ -238,7 +240,8 @@ class BatchedCP:
             for w in (2, 4, 8):
                 per_cu = lib().tla_batched_cp_max_blocks_per_cu(w)
                 for b in sorted({per_cu, max(1, per_cu // 2)}):
-                    candidates.append((w, b))
+                    for kt in (16, 32, 64):
+                        candidates.append((w, b, kt))
         f0 = self.random_init(seed)
         if converge:
             pol, n_sw, tl, kw = policy, int(max_sweeps), float(tol), dict(policy_kw)
@@ -246,10 +249,12 @@ class BatchedCP:
             pol, n_sw, tl, kw = "plain", int(sweeps), 0.0, {}
         out = self.make_outputs(n_sw, history=pol in ("learned", "probe"))
         table = []
-        for w, b in candidates:
+        for cand in candidates:
+            w, b = cand[0], cand[1]
+            kt = cand[2] if len(cand) > 2 else 32
             grid = self._grid(w, b)
             f = f0.clone()
-            self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, **kw)
+            self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, ktile=kt, **kw)
             torch.cuda.synchronize(self.device)
             best = float("inf")
             for _ in range(repeats):
@@ -257,11 +262,11 @@ class BatchedCP:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, **kw)
+                self.launch(f, out, policy=pol, max_sweeps=n_sw, tol=tl, waves=w, grid=grid, ktile=kt, **kw)
                 e1.record()
                 torch.cuda.synchronize(self.device)
                 best = min(best, e0.elapsed_time(e1))
-            table.append({"waves": w, "blocks_per_cu": b, "grid": grid, "ms": best})
+            table.append({"waves": w, "blocks_per_cu": b, "ktile": kt, "grid": grid, "ms": best})
         win = min(table, key=lambda t: t["ms"])
-        self.config = {"waves": win["waves"], "blocks_per_cu": win["blocks_per_cu"]}
+        self.config = {"waves": win["waves"], "blocks_per_cu": win["blocks_per_cu"], "ktile": win["ktile"]}
         return table

@@ -71,6 +71,7 @@ struct BatchArgs {
   int* n_accept_out;
   int* status_out;
   long long* ticks_out;
+  int kdepth;  // reduction tile of the MTTKRP: 16 * kdepth indices in flight per wave (1, 2 or 4)
 };
 
 struct Prob {
@@ -99,6 +100,7 @@ struct Smem {
   float hpart[kMlpH];
   double dscal[4];
   int iscal[4];
+  int kdepth;
 };
 
 // One instance per workgroup (LDS); every helper below refers to it directly,
@@ -185,8 +187,11 @@ __device__ void grams_all() {
 
 // M (part[0], a * 16 + r) = unfold(X, n) . KR(F_o, F_i) on MFMA, where o < i
 // are the two other modes.  Reduction index e = (index along o) * I_i +
-// (index along i); wave w takes a contiguous range of e.
-template <int NW, int T>
+// (index along i); wave w takes a contiguous range of e.  The reduction
+// tile: D chunks of 16 reduction indices are in flight per wave (the loads of
+// chunk c + D are issued after the MFMAs of chunk c); every D gives the same
+// MFMA sequence, so the same bits.
+template <int NW, int T, int D>
 __device__ void mttkrp_tiles(int n) {
   Smem<NW>& sm = g_sm<NW>;
   const Prob& p = sm.prob;
@@ -217,8 +222,6 @@ __device__ void mttkrp_tiles(int n) {
   for (int t = 0; t < T; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   int eo = (e0 + kq) / n_in, ei = (e0 + kq) - ((e0 + kq) / n_in) * n_in;
   const float* X = p.X;
-  // Two chunks of 16 reduction indices in flight: the loads of chunk c + 1 are
-  // issued before the MFMAs of chunk c.
   auto fetch = [&](int e, float (&xv)[4][T], float (&bv)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -234,21 +237,21 @@ __device__ void mttkrp_tiles(int n) {
       }
     }
   };
-  float xa[4][T], xb[4][T], ba[4], bb[4];
-  if (e0 < e1) fetch(e0, xa, ba);
-  for (int e = e0; e < e1; e += 32) {
-    const bool more = e + 16 < e1;
-    if (more) fetch(e + 16, xb, bb);
+  float xs[D][4][T], bs[D][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+  for (int d = 0; d < D; ++d)
+    if (e0 + 16 * d < e1) fetch(e0 + 16 * d, xs[d], bs[d]);
+  for (int e = e0; e < e1; e += 16 * D) {
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xa[u][t], ba[u], acc[t]);
-    if (!more) break;
-    if (e + 32 < e1) fetch(e + 32, xa, ba);
+    for (int d = 0; d < D; ++d) {
+      const int ec = e + 16 * d;
+      if (ec >= e1) break;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xb[u][t], bb[u], acc[t]);
+        for (int t = 0; t < T; ++t) acc[t] = mfma_f32_16x16x4(xs[d][u][t], bs[d][u], acc[t]);
+      if (ec + 16 * D < e1) fetch(ec + 16 * D, xs[d], bs[d]);
+    }
   }
 #pragma unroll
   for (int t = 0; t < T; ++t)
@@ -264,18 +267,29 @@ __device__ void mttkrp_tiles(int n) {
   __syncthreads();
 }
 
+template <int NW, int D>
+__device__ __forceinline__ void mttkrp_depth(int n, int T) {
+  if (T <= 1)
+    mttkrp_tiles<NW, 1, D>(n);
+  else if (T == 2)
+    mttkrp_tiles<NW, 2, D>(n);
+  else if (T == 3)
+    mttkrp_tiles<NW, 3, D>(n);
+  else
+    mttkrp_tiles<NW, 4, D>(n);
+}
+
 template <int NW>
 __device__ __noinline__ void mttkrp(int n) {
   const Prob& p = g_sm<NW>.prob;
   const int T = (p.dim(n) + 15) >> 4;
-  if (T <= 1)
-    mttkrp_tiles<NW, 1>(n);
-  else if (T == 2)
-    mttkrp_tiles<NW, 2>(n);
-  else if (T == 3)
-    mttkrp_tiles<NW, 3>(n);
+  const int D = g_sm<NW>.kdepth;
+  if (D == 1)
+    mttkrp_depth<NW, 1>(n, T);
+  else if (D == 4)
+    mttkrp_depth<NW, 4>(n, T);
   else
-    mttkrp_tiles<NW, 4>(n);
+    mttkrp_depth<NW, 2>(n, T);
 }
 
 // A_n = M V^-1 with V = Hadamard of the other two Grams; V^-1 by Gauss-Jordan
@@ -627,6 +641,7 @@ __device__ void run_problem(const BatchArgs& a, int pid) {
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) batched_cp_kernel(BatchArgs a) {
   Smem<NW>& sm = g_sm<NW>;
+  if (threadIdx.x == 0) sm.kdepth = a.kdepth;
   while (true) {
     if (threadIdx.x == 0) sm.iscal[1] = atomicAdd(a.queue, 1);
     __syncthreads();
@@ -667,7 +682,7 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
                                   float* hist_alpha, int* hist_acc, float* features, double* probe_err,
                                   int* sweeps_out, int* passes_out, double* err_out, int* n_extrap_out,
                                   int* n_accept_out, int* status_out, long long* ticks_out, int waves, int grid,
-                                  void* stream) {
+                                  int ktile, void* stream) {
   TLA_REQUIRE(n_problems >= 0, "batched_cp: n_problems = %d", n_problems);
   if (n_problems == 0) return TLA_OK;
   TLA_REQUIRE(X && x_off && shapes && ranks && factors && f_off && f_prev && f_save && order && queue,
@@ -678,6 +693,7 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
   TLA_REQUIRE(max_sweeps >= 1, "batched_cp: max_sweeps = %d", max_sweeps);
   TLA_REQUIRE(waves == 2 || waves == 4 || waves == 8, "batched_cp: waves = %d (2, 4 or 8)", waves);
   TLA_REQUIRE(grid >= 1, "batched_cp: grid = %d", grid);
+  TLA_REQUIRE(ktile == 16 || ktile == 32 || ktile == 64, "batched_cp: ktile = %d (16, 32 or 64)", ktile);
   if (policy == kLearned || policy == kProbe) {
     TLA_REQUIRE(hist_err && hist_alpha && hist_acc, "batched_cp: the extrapolating policies need the history arrays");
     TLA_REQUIRE(warmup >= 1, "batched_cp: warmup = %d", warmup);
@@ -722,6 +738,7 @@ extern "C" int tla_batched_cp_als(int n_problems, const float* X, const int64_t*
   a.n_accept_out = n_accept_out;
   a.status_out = status_out;
   a.ticks_out = ticks_out;
+  a.kdepth = ktile / 16;
   hipStream_t st = (hipStream_t)stream;
   reset_queue_kernel<<<1, 64, 0, st>>>(queue);
   TLA_LAUNCH_CHECK();

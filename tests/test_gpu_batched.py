@@ -256,9 +256,16 @@ def test_autotune_picks_a_candidate(cuda):
     b = _batch(cuda, probs)
     table = b.autotune(sweeps=3, repeats=1)
     assert len(table) >= 3
-    assert b.config["waves"] in (2, 4, 8)
+    assert b.config["waves"] in (2, 4, 8) and b.config["ktile"] in (16, 32, 64)
     f0 = b.random_init(0)
     f_a, o_a = b.run(f0, policy="plain", max_sweeps=2, tol=0.0)
+    # the reduction tile only changes how many loads are in flight: the
+    # MFMA sequence, hence every bit, is the same
+    w0 = b.config["waves"]
+    runs = [b.run(f0, policy="plain", max_sweeps=3, tol=0.0, waves=w0, grid=b._grid(w0), ktile=kt)
+            for kt in (16, 32, 64)]
+    for f_k, o_k in runs[1:]:
+        assert torch.equal(f_k, runs[0][0]) and torch.equal(o_k["error"], runs[0][1]["error"])
     for w in (2, 8):
         f_b, o_b = b.run(f0, policy="plain", max_sweeps=2, tol=0.0, waves=w, grid=b._grid(w))
         # the reduction split over waves changes the FP32 summation order only
@@ -272,7 +279,7 @@ def test_autotune_on_converging_runs(cuda):
                        alpha_fixed=1.0)
     assert len(table) >= 3 and all(t["ms"] > 0 for t in table)
     win = min(table, key=lambda t: t["ms"])
-    assert b.config == {"waves": win["waves"], "blocks_per_cu": win["blocks_per_cu"]}
+    assert b.config == {"waves": win["waves"], "blocks_per_cu": win["blocks_per_cu"], "ktile": win["ktile"]}
     # the tuned shape is used by later launches and solves the batch like any other shape
     _, o = b.run(b.random_init(0), policy="plain", max_sweeps=40, tol=1e-6)
     assert (o["status"].cpu().numpy() == 0).all()
