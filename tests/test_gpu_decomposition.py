@@ -114,6 +114,55 @@ def test_parafac_normalize_persistent_kernels(cuda, prec, dimtree):
     np.testing.assert_allclose(cp.weights.cpu().numpy(), w_ref, rtol=1e-3)
 
 
+@pytest.mark.parametrize("shape,rank", [((96, 80, 64), 32), ((96, 80, 64), 12), ((64, 72, 64), 20),
+                                        ((24, 32, 64, 64), 16), ((64, 40, 72, 32), 28)])
+def test_dimtree_sweep_matches_public_entry_points(cuda, shape, rank):
+    """One sweep of the native dimension-tree driver against the same sweep
+    composed from the public entry points, bit for bit, for every mode whose
+    MTTKRP is a rank-diagonal contraction of a shared partial (all four
+    modes of an order-4 tensor, modes 0 and 1 of an order-3 one).  The
+    driver takes each V^-1 from the side workgroup of the partial MTTKRP
+    launch or from the previous solve's finalising block; the public path
+    is tla_mttkrp_partial, tla_rankdiag_contract and tla_cp_solve inverting
+    V itself."""
+    from tensorly_amd import _lib
+    from tensorly_amd.cp_tensor import grams64
+    from tensorly_amd.decomposition._cp import CpAlsEngine
+    from tensorly_amd.tenalg import mttkrp_partial, rankdiag_contract
+    rng = np.random.default_rng(sum(shape) * 7 + rank)
+    x = torch.tensor(rng.random(shape), dtype=torch.float32, device=cuda)
+    init = [torch.tensor(rng.random((s, rank)), dtype=torch.float32, device=cuda) for s in shape]
+    eng = CpAlsEngine(x, torch.ones(rank, device=cuda), init, precision="bf16x3", dimtree=True, unit_weights=True)
+    assert eng.dimtree
+    eng.sweeps(1)
+    N = len(shape)
+    f = [a.clone() for a in init]
+    G = grams64(f)
+    L = _lib.lib()
+
+    def solve(mode, m):
+        rows = shape[mode]
+        gp, _keep = _lib.ptr_array(G)
+        g_out = torch.empty((rank, rank), dtype=torch.float64, device=cuda)
+        status = torch.zeros(1, dtype=torch.int32, device=cuda)
+        nb = L.tla_cp_solve_workspace_size(rows, rank)
+        ws = _lib.workspace(nb, cuda)
+        _lib.check(L.tla_cp_solve(_lib.F32, gp, N, mode, None, 0.0, _lib.ptr(m.contiguous()), rows, rank,
+                                  _lib.ptr(f[mode]), _lib.ptr(g_out), None, _lib.ptr(status), None, _lib.ptr(ws),
+                                  nb, _lib.stream(cuda)), "cp_solve")
+        G[mode] = g_out
+        assert int(status) == 0
+
+    for kb in range(0, N - 1, 2):
+        T = mttkrp_partial(x, (None, f), (kb, kb + 2), "bf16x3").reshape(shape[kb], shape[kb + 1], rank)
+        solve(kb, rankdiag_contract(T, f[kb + 1], 1))
+        solve(kb + 1, rankdiag_contract(T, f[kb], 0))
+    modes = range(N) if N == 4 else (0, 1)
+    for m in modes:
+        d = (eng.factors[m] - f[m]).abs().max().item()
+        assert torch.equal(eng.factors[m], f[m]), (shape, rank, m, d)
+
+
 def test_parafac_order4_dimtree(cuda):
     import tensorly_amd as tla
     rng = np.random.default_rng(5)
