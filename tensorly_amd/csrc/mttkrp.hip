@@ -695,12 +695,14 @@ __device__ __forceinline__ void stream_finish_tile(const StreamArgs& args, int64
 }
 
 // Chunks in flight per wave (measured: 5 gains nothing over 3 at K = 128
-// and K = 512).
+// and K = 512 for float tensors).  A bf16 tensor's chunk is half the
+// bytes, so it keeps more chunks in flight for the same bytes per wave.
 constexpr int kStreamDepth = 3;
+constexpr int kStreamDepthBf16 = 5;
 
 template <int PREC, typename XT, int RT, int EPI>
 __global__ __launch_bounds__(64 * kStreamWaves) void mttkrp_stream_kernel(StreamArgs args) {
-  constexpr int DEPTH = kStreamDepth;
+  constexpr int DEPTH = sizeof(XT) == 2 ? kStreamDepthBf16 : kStreamDepth;
   constexpr int RP = RT * 16;
   constexpr int NT = 64 * kStreamWaves;
   extern __shared__ __align__(16) unsigned char smem[];
