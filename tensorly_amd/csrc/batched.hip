@@ -192,7 +192,7 @@ __device__ void grams_all() {
 // chunk c + D are issued after the MFMAs of chunk c); every D gives the same
 // MFMA sequence, so the same bits.
 template <int NW, int T, int D>
-__device__ void mttkrp_tiles(int n) {
+__device__ __forceinline__ void mttkrp_tiles(int n) {
   Smem<NW>& sm = g_sm<NW>;
   const Prob& p = sm.prob;
   const int mo = (n == 0) ? 1 : 0;
