@@ -135,3 +135,25 @@ print("ok")
 ''' % ROOT
     res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
     assert res.returncode == 0 and "ok" in res.stdout, (res.returncode, res.stderr[-2000:])
+
+
+def test_batched_launch_shape_checked_on_the_host():
+    """tla_batched_cp_als rejects a launch shape outside what it is built
+    for (waves per workgroup 2 / 4 / 8, reduction tile 16 / 32 / 64 indices)
+    with TLA_ERR_ARG before anything reaches the device: the pointers here
+    are never dereferenced."""
+    from tensorly_amd import _lib
+    lib = _lib.lib()
+    dummy = ctypes.c_void_p(256)
+
+    def call(waves, ktile):
+        return lib.tla_batched_cp_als(
+            2, *([dummy] * 10), 0, 10, 1e-7, 3, None, 4.0, 16.0, None, 0, 0, 0, None, None, None, None, None,
+            *([dummy] * 6), None, waves, 256, ktile, None)
+
+    for ktile in (0, 8, 48, 128):
+        assert call(4, ktile) == _lib.ERR_ARG
+        assert b"ktile" in lib.tla_last_error()
+    for waves in (1, 3, 16):
+        assert call(waves, 32) == _lib.ERR_ARG
+        assert b"waves" in lib.tla_last_error()
